@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""bench.py — Lennard-Jones pair-interaction throughput on one MI355X.
+
+Workload (the north-star size of the reference's force path): FCC lattice of
+4*31^3 = 119,164 particles at density 1.0 in a periodic cubic box
+(L = 49.21), seeded jitter U(-0.05, 0.05), cutoff 3.0, search length 3.3,
+full Verlet list (16,895,170 entries = 8,447,585 unique pairs), dt = 0.01,
+AoS double4 positions and momenta resident in HBM. One "step" is one force
+evaluation p += F(q) * dt over the whole list (the body of the reference's
+measure() loop, cuda/force_cuda.cu:333-335).
+
+Prints ONE JSON line on stdout:
+  value          unique pairs * steps / GPU time of the FP64 kernel (Gpair/s),
+                 HIP events on the library's stream around exactly K steps
+  mixed          the same for the mixed-precision kernel, with its force error
+                 against the FP64 kernel (max and RMS, normwise)
+  roofline       algorithmic bytes per launch / average launch time, vs 8 TB/s
+  cpu_baseline   the oracle's force_sorted (the reference's CPU path restated:
+                 half list, j scatter) on one host core, a bounded sample of the
+                 same list
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpair-interactions/s FP64 vs mixed-FP32 at ≤1e-6 rel force err, N=119164"
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_ENTRY = 22  # sub 3, r2 5, r6 2, r14 2, numerator 2, divide 1, dt 1, accumulate 6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(q, box, half, dt, budget_s: float) -> dict:
+    """The oracle's force_sorted on one core over the same periodic list (a
+    bounded sample of steps, about `budget_s` seconds)."""
+    import oracle as O
+
+    csr = O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half")
+    t1 = O.time_force_sorted_aos(csr, q, dt, O.CL2, box, steps=1)
+    steps = max(1, min(1000, int(budget_s / max(t1, 1e-6))))
+    t = O.time_force_sorted_aos(csr, q, dt, O.CL2, box, steps=steps)
+    rate = half.n_entries * steps / t / 1e9
+    return {"value": round(rate, 6), "unit": "Gpair-interactions/s", "cores": 1, "kind": "port",
+            "sample": f"oracle force_sorted (reference cpu_ref/force_aos.cpp:192-224 restated, "
+                      f"-O2, minimum image) over the same N={half.n} half list "
+                      f"({half.n_entries} pairs), {steps} steps in {t:.2f} s, 1 thread",
+            "cpu": cpu_model(), "seconds": round(t, 3), "steps": steps}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cells", type=int, default=31)
+    ap.add_argument("--density", type=float, default=1.0)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: library default)")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: the force path is measured on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured configuration); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import lj_gpu_amd as G
+
+    dt = 0.01
+    t0 = time.time()
+    q, box = G.lattice_periodic(args.cells, args.density, 2)
+    full = G.makepair(q, box, G.LIST_FULL)
+    half = G.makepair(q, box, G.LIST_HALF)
+    n, e = full.n, full.n_entries
+    pairs = e // 2
+    log(f"setup: N={n} box={box:.4f} full entries={e} pairs={pairs} ({time.time() - t0:.1f}s)")
+
+    ctx = G.Context(0)
+    ctx.set_kernel_config(args.lanes, args.block)
+    ctx.set_list(full)
+    info = ctx.list_info()
+    dq = ctx.array(q)
+    dp = ctx.zeros(q.shape)
+
+    def timed(fn, steps):
+        ctx.sync()
+        ctx.timer_start()
+        fn(steps)
+        ms = ctx.timer_stop()
+        ctx.sync()
+        return ms
+
+    f64 = lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, box, s)
+    mixed = lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s, G.ACCUM_F64)
+
+    result = {}
+    ms_f64 = None
+    if args.only in ("all", "f64"):
+        f64(args.warmup)
+        ms_f64 = timed(f64, args.steps)
+        log(f"fp64: {args.steps} steps in {ms_f64:.3f} ms")
+    ms_mixed = None
+    if args.only in ("all", "mixed"):
+        mixed(args.warmup)
+        ms_mixed = timed(mixed, args.steps)
+        log(f"mixed: {args.steps} steps in {ms_mixed:.3f} ms")
+
+    # Force error of the mixed kernel against the FP64 kernel (one step, dt = 1).
+    dF = ctx.zeros(q.shape)
+    dM = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dF, 1.0, G.CL2, box, 1)
+    ctx.force_aos_mixed(dq, dM, 1.0, G.CL2, box, 1, G.ACCUM_F64)
+    ctx.sync()
+    F, M = dF.download()[:, :3], dM.download()[:, :3]
+    scale = float(np.abs(F).max())
+    err_max = float(np.abs(M - F).max() / scale)
+    err_rms = float(np.sqrt(((M - F) ** 2).mean()) / scale)
+
+    bytes_per_launch = 4 * e + 104 * n  # list + (nop, pointer) + q + p read/write, AoS double4
+    ms_main = ms_f64 if ms_f64 is not None else ms_mixed
+    per_launch_s = ms_main / 1e3 / args.steps
+    value = pairs * args.steps / (ms_main / 1e3) / 1e9
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_main / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if ms_f64 is not None else "f32/f64-mixed",
+        "data": "synthetic",
+        "config": {
+            "workload": f"LJ force, FCC 4*{args.cells}^3={n} particles, density {args.density}, "
+                        f"periodic box {box:.4f}, full Verlet list (cutoff 3.0, search 3.3), "
+                        f"dt {dt}, AoS double4",
+            "particles": n, "density": args.density, "box": round(box, 6),
+            "list_entries": e, "unique_pairs": pairs,
+            "lanes_per_particle": info.lanes_per_particle,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": round(per_launch_s * 1e6, 3),
+            "flop_rate_tflops": round(FLOP_PER_ENTRY * e / per_launch_s / 1e12, 3),
+        },
+        "mixed": None if ms_mixed is None else {
+            "value": round(pairs * args.steps / (ms_mixed / 1e3) / 1e9, 3),
+            "ms_per_step": round(ms_mixed / args.steps, 6),
+            "speedup_vs_f64": None if ms_f64 is None else round(ms_f64 / ms_mixed, 4),
+            "accum": "f64",
+            "max_rel_force_err_vs_f64": err_max,
+            "rms_rel_force_err_vs_f64": err_rms,
+        },
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(q, box, half, dt, args.cpu_seconds)
+        log(f"cpu baseline: {result['cpu_baseline']['value']} Gpair/s")
+    else:
+        result["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,232 @@
+/*
+ * ljgpu.h — C ABI of the MI355X (gfx950) Lennard-Jones force library.
+ *
+ * The hot path of kohnakagawa/lj_gpu: p += F(q) * dt over a Verlet neighbour
+ * list, with F the Lennard-Jones force (epsilon = sigma = 1, no shift),
+ * positions q fixed during a call. Every entry point below names the
+ * reference interface it replaces (paths relative to the reference checkout).
+ *
+ * Conventions
+ *   - Every function returns an int status: LJGPU_OK (0) or a negative
+ *     LJGPU_ERR_* code. Nothing aborts and nothing throws across this ABI;
+ *     ljgpu_last_error() returns a message for the calling thread.
+ *   - Particle counts are int32 (as in the reference), list lengths int64.
+ *   - Positions/momenta: AoS is double[N][4] (x, y, z, pad: the reference's
+ *     double4 / cpu_ref q[N][4]); SoA is three double[N] arrays (the
+ *     reference's q[X][N], q[Y][N], q[Z][N]).
+ *   - box > 0 selects a periodic cubic box of that side (minimum image on
+ *     every pair; positions must lie in [0, box)); box <= 0 is the
+ *     reference's open box.
+ *   - Momenta are accumulated into, never overwritten (the reference kernels'
+ *     p += F*dt semantics).
+ *   - A context owns one HIP stream (default priority). Calls on one context
+ *     are not thread-safe; use one context per host thread.
+ */
+#ifndef LJGPU_H
+#define LJGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define LJGPU_OK 0
+#define LJGPU_ERR_INVALID_ARGUMENT (-1)
+#define LJGPU_ERR_OUT_OF_MEMORY (-2)
+#define LJGPU_ERR_HIP (-3)
+#define LJGPU_ERR_BAD_LIST (-4)
+#define LJGPU_ERR_NO_LIST (-5)
+#define LJGPU_ERR_CAPACITY (-6)
+#define LJGPU_ERR_NO_DEVICE (-7)
+#define LJGPU_ERR_IO (-8)
+#define LJGPU_ERR_UNSUPPORTED (-9)
+
+/* ---- enums ------------------------------------------------------------ */
+/* List kinds: the reference's action-reaction half list (i < j, built with
+ * EN_ACTION_REACTION, cuda/force_cuda.cu:102-120; cpu_ref/force_aos.cpp:60-74)
+ * and its full list (every ordered pair i != j, cuda/force_cuda.cu:133-136). */
+#define LJGPU_LIST_HALF 0
+#define LJGPU_LIST_FULL 1
+
+/* Where caller buffers live. */
+#define LJGPU_MEM_HOST 0
+#define LJGPU_MEM_DEVICE 1
+
+/* Per-particle accumulator of the mixed-precision kernels. */
+#define LJGPU_ACCUM_F64 0
+#define LJGPU_ACCUM_KAHAN_F32 1
+
+/* How a half list is evaluated on the device. */
+#define LJGPU_HALF_AUTO 0       /* library choice (autotuned)               */
+#define LJGPU_HALF_ATOMIC 1     /* scatter to j with FP64 atomics, like the
+                                   reference's *_with_aar kernels
+                                   (cuda/kernel.cuh:238-469); summation
+                                   order is not deterministic               */
+#define LJGPU_HALF_EXPAND 2     /* expand to the full list once at
+                                   set-list time; deterministic             */
+
+typedef struct ljgpu_ctx ljgpu_ctx;
+
+typedef struct ljgpu_list_info {
+  int32_t particle_number;
+  int32_t kind;            /* LJGPU_LIST_HALF or LJGPU_LIST_FULL (as given) */
+  int64_t n_entries;       /* entries of the list as given                  */
+  int64_t n_pairs;         /* unique pairs (half: n_entries; full: /2)      */
+  int32_t max_partners;    /* longest row                                   */
+  int32_t min_partners;    /* shortest row                                  */
+  int32_t half_mode;       /* LJGPU_HALF_ATOMIC / LJGPU_HALF_EXPAND in use  */
+  int32_t lanes_per_particle; /* kernel configuration in use                */
+} ljgpu_list_info;
+
+/* ---- context, errors, device memory ----------------------------------- */
+/* Replaces the reference's global cuda_ptr<T> buffers and implicit default
+ * stream (cuda/cuda_ptr.cuh:10-104, cuda/force_cuda.cu:24-30,265-289). */
+int ljgpu_device_count(int* count);
+int ljgpu_create(int device, ljgpu_ctx** out);
+int ljgpu_destroy(ljgpu_ctx* ctx);
+int ljgpu_sync(ljgpu_ctx* ctx);
+const char* ljgpu_last_error(void);
+const char* ljgpu_version(void);
+
+/* Plain device memory (cuda_ptr::allocate / host2dev / dev2host / set_val,
+ * cuda/cuda_ptr.cuh:37-87). Copies are synchronous with respect to the host
+ * and ordered on the context stream. */
+int ljgpu_malloc(ljgpu_ctx* ctx, size_t bytes, void** out);
+int ljgpu_free(ljgpu_ctx* ctx, void* ptr);
+int ljgpu_memcpy_htod(ljgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ljgpu_memcpy_dtoh(ljgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ljgpu_memcpy_dtod(ljgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ljgpu_memset(ljgpu_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* Stream timer: HIP events recorded on the context stream around the work
+ * issued between start and stop (the reference's gettimeofday pair around
+ * measure()'s loop, cuda/force_cuda.cu:330-341). */
+int ljgpu_timer_start(ljgpu_ctx* ctx);
+int ljgpu_timer_stop(ljgpu_ctx* ctx, float* elapsed_ms);
+
+/* ---- neighbour list --------------------------------------------------- */
+/* Install a CSR neighbour list: number_of_partners[N], pointer[N] (row
+ * start), sorted_list[n_entries] — the arrays makepair()/sortpair() build
+ * (cuda/force_cuda.cu:122-163; cpu_ref/force_aos.cpp:100-140) and every
+ * force_kernel_* reads (cuda/kernel.cuh:5-13). The list is validated as the
+ * reference's check_loadedpair() does (cuda/force_cuda.cu:183-201, plus
+ * j != i, row ends inside the list and half-list j > i) and copied; device
+ * inputs are copied too, so the caller keeps ownership. Internal layouts
+ * (expanded full list for LJGPU_HALF_EXPAND) are derived here, once. */
+int ljgpu_set_list_csr(ljgpu_ctx* ctx, int32_t particle_number,
+                       const int32_t* number_of_partners, const int32_t* pointer,
+                       const int32_t* sorted_list, int64_t n_entries, int kind, int mem);
+
+/* Install the reference's ELL ("transposed") list: entry k of row i at
+ * transposed_list[i + k*N], rows zero-padded to `width`
+ * (make_transposed_pairlist, cuda/force_cuda.cu:229-240; read by
+ * force_kernel_memopt2 and relatives, cuda/kernel.cuh:102-236,471-819). */
+int ljgpu_set_list_transposed(ljgpu_ctx* ctx, int32_t particle_number,
+                              const int32_t* number_of_partners,
+                              const int32_t* transposed_list, int32_t width, int kind,
+                              int mem);
+
+int ljgpu_get_list_info(ljgpu_ctx* ctx, ljgpu_list_info* info);
+
+/* Copy the installed list back as CSR (caller buffers on the host). */
+int ljgpu_get_list_csr(ljgpu_ctx* ctx, int32_t* number_of_partners, int32_t* pointer,
+                       int32_t* sorted_list, int64_t capacity);
+
+/* Half-list evaluation mode (LJGPU_HALF_*); takes effect on the next
+ * ljgpu_set_list_* call. */
+int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode);
+
+/* Kernel configuration: lanes cooperating on one i-particle (1 selects the
+ * thread-per-particle ELL kernel; 4, 8, 16, 32 or 64 the CSR
+ * wavefront-segment kernel) and the workgroup size (64..1024, multiple of
+ * 64). 0 keeps the current value. The reference exposes only the block size
+ * (argv[1], cuda/force_cuda.cu:377-383). */
+int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_size);
+
+/* ---- forces ------------------------------------------------------------ */
+/* p += F(q) * dt over the installed list, repeated `steps` times (the
+ * reference's measure() loop of 100 launches, cuda/force_cuda.cu:319-342;
+ * cpu_ref/force_aos.cpp:592-601). q and p are device pointers. FP64
+ * throughout: the pair term of cpu_ref/force_aos.cpp:176-189 and
+ * cuda/kernel.cuh:22-31. Asynchronous: call ljgpu_sync before reading p. */
+int ljgpu_force_aos_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                        double box, int32_t steps);
+int ljgpu_force_soa_f64(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
+                        double* px, double* py, double* pz, double dt, double cl2,
+                        double box, int32_t steps);
+
+/* Mixed precision: pair distances and r2 in FP64, r2 rounded once to FP32,
+ * r^-6 / r^-14 and the force scalar in FP32, accumulation in FP64 or
+ * Kahan-compensated FP32 (accum = LJGPU_ACCUM_*), momentum update in FP64.
+ * No reference counterpart (the reference allocates float3/float4 buffers,
+ * cuda/force_cuda.cu:24-25, but never evaluates forces with them). */
+int ljgpu_force_aos_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                          double box, int32_t steps, int accum);
+int ljgpu_force_soa_mixed(ljgpu_ctx* ctx, const double* qx, const double* qy,
+                          const double* qz, double* px, double* py, double* pz, double dt,
+                          double cl2, double box, int32_t steps, int accum);
+
+/* One launch with exactly the reference kernel's argument list
+ * (force_kernel_*(q, p, particle_number, dt, CL2, sorted_list,
+ * number_of_partners, pointer), cuda/kernel.cuh:5-13), all device pointers,
+ * AoS double4, open box. kind selects full-list (p[i] only) or half-list
+ * (atomic scatter) semantics. No list is installed or validated: every index
+ * read is clamped into range, so a malformed list gives wrong numbers, never
+ * an out-of-bounds access. */
+int ljgpu_force_kernel_csr_f64(ljgpu_ctx* ctx, const double* q, double* p,
+                               int32_t particle_number, double dt, double cl2,
+                               const int32_t* sorted_list, const int32_t* number_of_partners,
+                               const int32_t* pointer, int64_t n_entries, int kind);
+
+/* ---- host-side setup (CPU only, no device needed) ----------------------- */
+/* The reference's init(): FCC lattice in an open box of side box_length,
+ * jitter U(0, 0.1) from std::mt19937(2), AoS double[N][4].
+ * cpu_ref/force_aos.cpp:43-51,142-169; cuda/force_cuda.cu:47-94. */
+int ljgpu_lattice_reference(double density, double box_length, double* q, int32_t capacity,
+                            int32_t* particle_number);
+
+/* Periodic FCC lattice of 4*cells^3 particles, same lattice constant and
+ * particle order as init(), side L = cells * (4/density)^(1/3), jitter
+ * U(jitter_lo, jitter_hi) from std::mt19937(seed), wrapped into [0, L).
+ * No reference counterpart (the reference has no periodic box). */
+int ljgpu_lattice_periodic(int32_t cells, double density, uint32_t seed, double jitter_lo,
+                           double jitter_hi, double* q, int32_t capacity, double* box);
+
+/* Verlet list with the reference's exact pair set and order: predicate
+ * r2 < search_length^2 with r2 = (dx*dx + dy*dy) + dz*dz unfused, rows in
+ * ascending i, partners ascending (makepair + sortpair,
+ * cpu_ref/force_aos.cpp:100-140; full list cuda/force_cuda.cu:122-163).
+ * Built with a cell list in O(N) instead of the reference's O(N^2) scan.
+ * Writes at most `capacity` entries; *n_entries receives the true count
+ * (LJGPU_ERR_CAPACITY if it exceeds capacity). */
+int ljgpu_makepair_host(int32_t particle_number, const double* q, double box,
+                        double search_length, int kind, int32_t* number_of_partners,
+                        int32_t* pointer, int32_t* sorted_list, int64_t capacity,
+                        int64_t* n_entries);
+
+/* check_loadedpair() (cuda/force_cuda.cu:183-201) made strict: row counts,
+ * row starts and ends inside the list, 0 <= j < N, j != i, and j > i for a
+ * half list. */
+int ljgpu_validate_csr(int32_t particle_number, const int32_t* number_of_partners,
+                       const int32_t* pointer, const int32_t* sorted_list, int64_t n_entries,
+                       int kind);
+
+/* random_shfl(): shuffle every row with one std::mt19937(seed) stream (seed
+ * 10 in the reference), cuda/force_cuda.cu:255-263. */
+int ljgpu_random_shfl(int32_t particle_number, const int32_t* number_of_partners,
+                      const int32_t* pointer, int32_t* sorted_list, uint32_t seed);
+
+/* make_transposed_pairlist() (cuda/force_cuda.cu:229-240): ELL with stride N,
+ * zero padded to width (>= the longest row). */
+int ljgpu_make_transposed(int32_t particle_number, const int32_t* number_of_partners,
+                          const int32_t* pointer, const int32_t* sorted_list, int32_t width,
+                          int32_t* transposed_list);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LJGPU_H */

@@ -1,0 +1,410 @@
+"""lj_gpu_amd — MI355X-native Lennard-Jones force evaluation over a Verlet list.
+
+Python host mirror of the reference harness (kohnakagawa/lj_gpu,
+cuda/force_cuda.cu and cpu_ref/force_aos.cpp) over the C ABI of
+``libljgpu.so`` (include/ljgpu.h). The reference's host steps map as:
+
+    init()                       -> lattice_reference() / lattice_periodic()
+    makepair() + sortpair()      -> makepair()            (cell list, exact pair set)
+    random_shfl()                -> random_shfl()
+    make_transposed_pairlist()   -> make_transposed()
+    cuda_ptr<T>                  -> DeviceArray
+    measure(kernel, ...)         -> Context.force_* (+ Context.timer_*)
+    check_loadedpair()           -> validate_csr()
+
+All compute runs in the HIP kernels of libljgpu.so; if the library is
+missing, importing the device API raises (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libljgpu.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "ljgpu.h")
+
+# Status codes and enums (include/ljgpu.h).
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_OUT_OF_MEMORY = -2
+ERR_HIP = -3
+ERR_BAD_LIST = -4
+ERR_NO_LIST = -5
+ERR_CAPACITY = -6
+ERR_NO_DEVICE = -7
+ERR_IO = -8
+ERR_UNSUPPORTED = -9
+
+LIST_HALF = 0
+LIST_FULL = 1
+MEM_HOST = 0
+MEM_DEVICE = 1
+ACCUM_F64 = 0
+ACCUM_KAHAN_F32 = 1
+HALF_AUTO = 0
+HALF_ATOMIC = 1
+HALF_EXPAND = 2
+
+# The reference problem's constants (cpu_ref/force_aos.cpp:10-14,31-33).
+CUTOFF_LENGTH = 3.0
+SEARCH_LENGTH = 3.3
+CL2 = CUTOFF_LENGTH * CUTOFF_LENGTH
+REF_DT = 0.001
+REF_BOX_LENGTH = 50.0
+
+
+class LJGPUError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ljgpu error {code}: {msg}")
+        self.code = code
+
+
+class ListInfo(C.Structure):
+    _fields_ = [("particle_number", C.c_int32), ("kind", C.c_int32), ("n_entries", C.c_int64),
+                ("n_pairs", C.c_int64), ("max_partners", C.c_int32), ("min_partners", C.c_int32),
+                ("half_mode", C.c_int32), ("lanes_per_particle", C.c_int32)]
+
+
+_lib = None
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_d = C.c_double
+
+
+def lib() -> C.CDLL:
+    """Load libljgpu.so (built by lj_gpu_amd/build.py); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m lj_gpu_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "ljgpu_device_count": [C.POINTER(C.c_int)],
+        "ljgpu_create": [C.c_int, C.POINTER(_vp)],
+        "ljgpu_destroy": [_vp],
+        "ljgpu_sync": [_vp],
+        "ljgpu_malloc": [_vp, C.c_size_t, C.POINTER(_vp)],
+        "ljgpu_free": [_vp, _vp],
+        "ljgpu_memcpy_htod": [_vp, _vp, _vp, C.c_size_t],
+        "ljgpu_memcpy_dtoh": [_vp, _vp, _vp, C.c_size_t],
+        "ljgpu_memcpy_dtod": [_vp, _vp, _vp, C.c_size_t],
+        "ljgpu_memset": [_vp, _vp, C.c_int, C.c_size_t],
+        "ljgpu_timer_start": [_vp],
+        "ljgpu_timer_stop": [_vp, C.POINTER(C.c_float)],
+        "ljgpu_set_list_csr": [_vp, _i32, _vp, _vp, _vp, _i64, C.c_int, C.c_int],
+        "ljgpu_set_list_transposed": [_vp, _i32, _vp, _vp, _i32, C.c_int, C.c_int],
+        "ljgpu_get_list_info": [_vp, C.POINTER(ListInfo)],
+        "ljgpu_get_list_csr": [_vp, _vp, _vp, _vp, _i64],
+        "ljgpu_set_half_mode": [_vp, C.c_int],
+        "ljgpu_set_kernel_config": [_vp, C.c_int, C.c_int],
+        "ljgpu_force_aos_f64": [_vp, _vp, _vp, _d, _d, _d, _i32],
+        "ljgpu_force_soa_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32],
+        "ljgpu_force_aos_mixed": [_vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
+        "ljgpu_force_soa_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
+        "ljgpu_force_kernel_csr_f64": [_vp, _vp, _vp, _i32, _d, _d, _vp, _vp, _vp, _i64, C.c_int],
+        "ljgpu_lattice_reference": [_d, _d, _vp, _i32, C.POINTER(_i32)],
+        "ljgpu_lattice_periodic": [_i32, _d, C.c_uint32, _d, _d, _vp, _i32, C.POINTER(_d)],
+        "ljgpu_makepair_host": [_i32, _vp, _d, _d, C.c_int, _vp, _vp, _vp, _i64, C.POINTER(_i64)],
+        "ljgpu_validate_csr": [_i32, _vp, _vp, _vp, _i64, C.c_int],
+        "ljgpu_random_shfl": [_i32, _vp, _vp, _vp, C.c_uint32],
+        "ljgpu_make_transposed": [_i32, _vp, _vp, _vp, _i32, _vp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.ljgpu_last_error.restype = C.c_char_p
+    L.ljgpu_last_error.argtypes = []
+    L.ljgpu_version.restype = C.c_char_p
+    L.ljgpu_version.argtypes = []
+    _lib = L
+    return L
+
+
+def header_functions() -> list[str]:
+    """Names of every function declared in include/ljgpu.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(ljgpu_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().ljgpu_last_error()
+        raise LJGPUError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    if a is None:
+        return None
+    assert a.flags.c_contiguous
+    return a.ctypes.data if a.size else None
+
+
+# ---------------------------------------------------------------------------
+# Host-side setup (the reference's init / makepair / sortpair / random_shfl)
+# ---------------------------------------------------------------------------
+@dataclass
+class CSR:
+    """A Verlet list in the reference's CSR form (number_of_partners,
+    pointer, sorted_list; cuda/force_cuda.cu:28,146-162)."""
+    number_of_partners: np.ndarray
+    pointer: np.ndarray
+    sorted_list: np.ndarray
+    kind: int
+
+    @property
+    def n(self) -> int:
+        return int(self.number_of_partners.shape[0])
+
+    @property
+    def n_entries(self) -> int:
+        return int(self.sorted_list.shape[0])
+
+    @property
+    def n_pairs(self) -> int:
+        return self.n_entries if self.kind == LIST_HALF else self.n_entries // 2
+
+
+def lattice_reference(density: float = 0.5, box_length: float = REF_BOX_LENGTH) -> np.ndarray:
+    """The reference's init(): open-box FCC lattice, (N, 4) float64."""
+    s = 1.0 / (density * 0.25) ** (1.0 / 3.0)
+    cap = 4 * (int(box_length / s) + 1) ** 3
+    q = np.zeros((cap, 4), dtype=np.float64)
+    n = _i32(0)
+    _check(lib().ljgpu_lattice_reference(density, box_length, _ptr(q), cap, C.byref(n)))
+    return q[: n.value].copy()
+
+
+def lattice_periodic(cells: int, density: float, seed: int = 2,
+                     jitter: tuple[float, float] = (-0.05, 0.05)) -> tuple[np.ndarray, float]:
+    """Periodic FCC lattice of 4*cells^3 particles; returns (q (N, 4), box)."""
+    n = 4 * cells ** 3
+    q = np.zeros((n, 4), dtype=np.float64)
+    box = _d(0.0)
+    _check(lib().ljgpu_lattice_periodic(cells, density, seed, jitter[0], jitter[1], _ptr(q), n,
+                                        C.byref(box)))
+    return q, box.value
+
+
+def makepair(q: np.ndarray, box: float = 0.0, kind: int = LIST_HALF,
+             search_length: float = SEARCH_LENGTH) -> CSR:
+    """Verlet list with the reference's exact pair set and CSR order."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.shape[0]
+    nop = np.zeros(n, np.int32)
+    ptr = np.zeros(n, np.int32)
+    cap = _i64(0)
+    rc = lib().ljgpu_makepair_host(n, _ptr(q), box, search_length, kind, _ptr(nop), _ptr(ptr),
+                                   None, 0, C.byref(cap))
+    if rc not in (OK, ERR_CAPACITY):
+        _check(rc)
+    lst = np.zeros(max(cap.value, 1), np.int32)
+    e = _i64(0)
+    _check(lib().ljgpu_makepair_host(n, _ptr(q), box, search_length, kind, _ptr(nop), _ptr(ptr),
+                                     _ptr(lst), lst.size, C.byref(e)))
+    return CSR(nop, ptr, lst[: e.value].copy(), kind)
+
+
+def validate_csr(csr: CSR) -> None:
+    _check(lib().ljgpu_validate_csr(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
+                                    _ptr(csr.sorted_list), csr.n_entries, csr.kind))
+
+
+def random_shfl(csr: CSR, seed: int = 10) -> CSR:
+    lst = csr.sorted_list.copy()
+    _check(lib().ljgpu_random_shfl(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
+                                   _ptr(lst), seed))
+    return CSR(csr.number_of_partners.copy(), csr.pointer.copy(), lst, csr.kind)
+
+
+def make_transposed(csr: CSR, width: int | None = None) -> np.ndarray:
+    if width is None:
+        width = int(csr.number_of_partners.max()) if csr.n else 0
+    out = np.zeros(max(csr.n * width, 1), np.int32)
+    _check(lib().ljgpu_make_transposed(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
+                                       _ptr(csr.sorted_list), width, _ptr(out)))
+    return out[: csr.n * width]
+
+
+# ---------------------------------------------------------------------------
+# Device side
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """Device buffer owned by a Context (the reference's cuda_ptr<T>)."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        _check(lib().ljgpu_malloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        _check(lib().ljgpu_memcpy_htod(self.ctx.handle, self.ptr, _ptr(a), self.nbytes))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().ljgpu_memcpy_dtoh(self.ctx.handle, _ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        _check(lib().ljgpu_memset(self.ctx.handle, self.ptr, 0, self.nbytes))
+
+    def free(self) -> None:
+        if self.ptr is not None and self.ctx.handle is not None:
+            lib().ljgpu_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (one HIP stream) of libljgpu."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().ljgpu_create(device, C.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().ljgpu_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # memory
+    def array(self, a: np.ndarray) -> DeviceArray:
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def empty(self, shape, dtype=np.float64) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def zeros(self, shape, dtype=np.float64) -> DeviceArray:
+        d = DeviceArray(self, shape, dtype)
+        d.zero()
+        return d
+
+    def sync(self) -> None:
+        _check(lib().ljgpu_sync(self.handle))
+
+    # configuration
+    def set_kernel_config(self, lanes: int = 0, block: int = 0) -> None:
+        _check(lib().ljgpu_set_kernel_config(self.handle, lanes, block))
+
+    def set_half_mode(self, mode: int) -> None:
+        _check(lib().ljgpu_set_half_mode(self.handle, mode))
+
+    # lists
+    def set_list(self, csr: CSR) -> None:
+        _check(lib().ljgpu_set_list_csr(self.handle, csr.n, _ptr(csr.number_of_partners),
+                                        _ptr(csr.pointer), _ptr(csr.sorted_list),
+                                        csr.n_entries, csr.kind, MEM_HOST))
+
+    def set_list_device(self, n: int, nop: DeviceArray, ptr: DeviceArray, lst: DeviceArray,
+                        n_entries: int, kind: int) -> None:
+        _check(lib().ljgpu_set_list_csr(self.handle, n, nop.ptr, ptr.ptr, lst.ptr, n_entries,
+                                        kind, MEM_DEVICE))
+
+    def set_list_transposed(self, n: int, nop: np.ndarray, transposed: np.ndarray, width: int,
+                            kind: int) -> None:
+        _check(lib().ljgpu_set_list_transposed(self.handle, n, _ptr(nop), _ptr(transposed),
+                                               width, kind, MEM_HOST))
+
+    def list_info(self) -> ListInfo:
+        info = ListInfo()
+        _check(lib().ljgpu_get_list_info(self.handle, C.byref(info)))
+        return info
+
+    def get_list(self) -> CSR:
+        info = self.list_info()
+        n = info.particle_number
+        # The evaluated list can be the expanded full list of a half input.
+        nop = np.zeros(n, np.int32)
+        ptr = np.zeros(n, np.int32)
+        cap = max(2 * info.n_entries, 1)
+        lst = np.zeros(cap, np.int32)
+        _check(lib().ljgpu_get_list_csr(self.handle, _ptr(nop), _ptr(ptr), _ptr(lst), cap))
+        total = int(nop.sum())
+        kind = LIST_FULL if (info.kind == LIST_FULL or info.half_mode == HALF_EXPAND) else LIST_HALF
+        return CSR(nop, ptr, lst[:total].copy(), kind)
+
+    # forces
+    def force_aos_f64(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
+                      box: float = 0.0, steps: int = 1) -> None:
+        _check(lib().ljgpu_force_aos_f64(self.handle, q.ptr, p.ptr, dt, cl2, box, steps))
+
+    def force_soa_f64(self, q3: list, p3: list, dt: float, cl2: float = CL2, box: float = 0.0,
+                      steps: int = 1) -> None:
+        _check(lib().ljgpu_force_soa_f64(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, p3[0].ptr,
+                                         p3[1].ptr, p3[2].ptr, dt, cl2, box, steps))
+
+    def force_aos_mixed(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
+                        box: float = 0.0, steps: int = 1, accum: int = ACCUM_F64) -> None:
+        _check(lib().ljgpu_force_aos_mixed(self.handle, q.ptr, p.ptr, dt, cl2, box, steps, accum))
+
+    def force_soa_mixed(self, q3: list, p3: list, dt: float, cl2: float = CL2, box: float = 0.0,
+                        steps: int = 1, accum: int = ACCUM_F64) -> None:
+        _check(lib().ljgpu_force_soa_mixed(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, p3[0].ptr,
+                                           p3[1].ptr, p3[2].ptr, dt, cl2, box, steps, accum))
+
+    def force_kernel_csr_f64(self, q: DeviceArray, p: DeviceArray, n: int, dt: float, cl2: float,
+                             sorted_list: DeviceArray, nop: DeviceArray, ptr: DeviceArray,
+                             n_entries: int, kind: int) -> None:
+        _check(lib().ljgpu_force_kernel_csr_f64(self.handle, q.ptr, p.ptr, n, dt, cl2,
+                                                sorted_list.ptr, nop.ptr, ptr.ptr, n_entries,
+                                                kind))
+
+    # timing
+    def timer_start(self) -> None:
+        _check(lib().ljgpu_timer_start(self.handle))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float(0.0)
+        _check(lib().ljgpu_timer_stop(self.handle, C.byref(ms)))
+        return float(ms.value)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    rc = lib().ljgpu_device_count(C.byref(c))
+    return c.value if rc == OK else 0
+
+
+def measure(ctx: Context, q: DeviceArray, p: DeviceArray, dt: float, steps: int = 100,
+            cl2: float = CL2, box: float = 0.0, mixed: bool = False,
+            accum: int = ACCUM_F64) -> float:
+    """The reference's measure(): `steps` force calls on the installed list,
+    timed on the context stream; returns milliseconds (cuda/force_cuda.cu:319-342)."""
+    ctx.timer_start()
+    if mixed:
+        ctx.force_aos_mixed(q, p, dt, cl2, box, steps, accum)
+    else:
+        ctx.force_aos_f64(q, p, dt, cl2, box, steps)
+    return ctx.timer_stop()

@@ -1,0 +1,346 @@
+// host.cpp — host-side setup of the ljgpu library: lattices, the Verlet-list
+// build (cell list, exact reference pair set), list validation, row shuffle
+// and the ELL transpose. Plain C++17, compiled with -ffp-contract=off so the
+// pair predicate r2 = (dx*dx + dy*dy) + dz*dz is evaluated unfused, exactly
+// as the reference's makepair writes it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ljgpu_internal.h"
+
+namespace ljgpu {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+// Minimum image for a periodic box of side `box` (box <= 0: open box).
+static inline double min_image(double d, double box) {
+  if (box > 0.0) {
+    const double h = 0.5 * box;
+    if (d > h) d -= box;
+    else if (d < -h) d += box;
+  }
+  return d;
+}
+
+int validate_csr(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,
+                 int64_t n_entries, int kind) {
+  if (n < 0) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "particle_number < 0");
+  if (n_entries < 0) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "n_entries < 0");
+  if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "kind must be LJGPU_LIST_HALF or LJGPU_LIST_FULL");
+  if (n > 0 && (!nop || !ptr)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null list array");
+  if (n_entries > 0 && !list) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null sorted_list");
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t np = nop[i], kp = ptr[i];
+    if (np < 0 || np >= std::max<int64_t>(n, 1))
+      return set_error(LJGPU_ERR_BAD_LIST,
+                       "number_of_partners[" + std::to_string(i) + "] = " + std::to_string(np));
+    if (kp < 0 || kp + np > n_entries)
+      return set_error(LJGPU_ERR_BAD_LIST, "pointer[" + std::to_string(i) + "] = " +
+                                               std::to_string(kp) + " (row leaves the list)");
+    for (int64_t k = 0; k < np; k++) {
+      const int32_t j = list[kp + k];
+      if (j < 0 || j >= n)
+        return set_error(LJGPU_ERR_BAD_LIST, "sorted_list[" + std::to_string(kp + k) +
+                                                 "] = " + std::to_string(j) + " out of range");
+      if (j == i)
+        return set_error(LJGPU_ERR_BAD_LIST, "row " + std::to_string(i) + " lists itself");
+      if (kind == LJGPU_LIST_HALF && j < i)
+        return set_error(LJGPU_ERR_BAD_LIST, "half list row " + std::to_string(i) +
+                                                 " holds j = " + std::to_string(j) + " < i");
+    }
+  }
+  return LJGPU_OK;
+}
+
+void expand_half_to_full(const HostCSR& half, HostCSR* full) {
+  const int32_t n = half.n;
+  full->n = n;
+  std::vector<int32_t> lower(n, 0);
+  for (int32_t i = 0; i < n; i++)
+    for (int32_t k = 0; k < half.nop[i]; k++) lower[half.list[half.ptr[i] + k]]++;
+  full->nop.assign(n, 0);
+  full->ptr.assign(n, 0);
+  int64_t pos = 0;
+  for (int32_t i = 0; i < n; i++) {
+    full->nop[i] = lower[i] + half.nop[i];
+    full->ptr[i] = static_cast<int32_t>(pos);
+    pos += full->nop[i];
+  }
+  full->list.assign(static_cast<size_t>(pos), 0);
+  std::vector<int32_t> cursor(n, 0);
+  for (int32_t i = 0; i < n; i++) {
+    const int32_t kp = half.ptr[i];
+    for (int32_t k = 0; k < half.nop[i]; k++) {
+      const int32_t j = half.list[kp + k];
+      full->list[full->ptr[j] + cursor[j]++] = i;  // i < j, visited in ascending i
+    }
+  }
+  for (int32_t i = 0; i < n; i++) {
+    std::memcpy(full->list.data() + full->ptr[i] + lower[i],
+                half.list.data() + half.ptr[i], sizeof(int32_t) * half.nop[i]);
+  }
+}
+
+}  // namespace ljgpu
+
+using namespace ljgpu;
+
+extern "C" {
+
+const char* ljgpu_last_error(void) { return g_last_error.c_str(); }
+const char* ljgpu_version(void) { return "ljgpu 0.1 (gfx950)"; }
+
+// cpu_ref/force_aos.cpp:43-51,142-169: FCC lattice, open box, U(0,0.1) jitter.
+int ljgpu_lattice_reference(double density, double box_length, double* q, int32_t capacity,
+                            int32_t* particle_number) {
+  if (!(density > 0.0) || !(box_length > 0.0) || !q || !particle_number)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "lattice_reference: bad argument");
+  const double s = 1.0 / std::pow(density * 0.25, 1.0 / 3.0);
+  const double hs = s * 0.5;
+  const int sx = static_cast<int>(box_length / s);
+  const int64_t total = 4LL * sx * sx * sx;
+  if (total > capacity) {
+    *particle_number = static_cast<int32_t>(std::min<int64_t>(total, INT32_MAX));
+    return set_error(LJGPU_ERR_CAPACITY, "lattice_reference: capacity too small");
+  }
+  std::mt19937 mt(2);
+  std::uniform_real_distribution<double> ud(0.0, 0.1);
+  int32_t n = 0;
+  auto add = [&](double x, double y, double z) {
+    const double jx = ud(mt);
+    const double jy = ud(mt);
+    const double jz = ud(mt);
+    double* r = q + 4 * static_cast<int64_t>(n);
+    r[0] = x + jx;
+    r[1] = y + jy;
+    r[2] = z + jz;
+    r[3] = 0.0;
+    n++;
+  };
+  for (int iz = 0; iz < sx; iz++)
+    for (int iy = 0; iy < sx; iy++)
+      for (int ix = 0; ix < sx; ix++) {
+        const double x = ix * s, y = iy * s, z = iz * s;
+        add(x, y, z);
+        add(x, y + hs, z + hs);
+        add(x + hs, y, z + hs);
+        add(x + hs, y + hs, z);
+      }
+  *particle_number = n;
+  return LJGPU_OK;
+}
+
+int ljgpu_lattice_periodic(int32_t cells, double density, uint32_t seed, double jitter_lo,
+                           double jitter_hi, double* q, int32_t capacity, double* box) {
+  if (cells <= 0 || !(density > 0.0) || !q || !box || !(jitter_hi >= jitter_lo))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "lattice_periodic: bad argument");
+  const int64_t total = 4LL * cells * cells * cells;
+  if (total > capacity) return set_error(LJGPU_ERR_CAPACITY, "lattice_periodic: capacity too small");
+  const double s = 1.0 / std::pow(density * 0.25, 1.0 / 3.0);
+  const double hs = s * 0.5;
+  const double L = cells * s;
+  std::mt19937 mt(seed);
+  std::uniform_real_distribution<double> ud(jitter_lo, jitter_hi);
+  auto wrap = [L](double v) {
+    if (v < 0.0) v += L;
+    if (v >= L) v -= L;
+    return v;
+  };
+  int64_t n = 0;
+  auto add = [&](double x, double y, double z) {
+    const double jx = ud(mt);
+    const double jy = ud(mt);
+    const double jz = ud(mt);
+    double* r = q + 4 * n;
+    r[0] = wrap(x + jx);
+    r[1] = wrap(y + jy);
+    r[2] = wrap(z + jz);
+    r[3] = 0.0;
+    n++;
+  };
+  for (int iz = 0; iz < cells; iz++)
+    for (int iy = 0; iy < cells; iy++)
+      for (int ix = 0; ix < cells; ix++) {
+        const double x = ix * s, y = iy * s, z = iz * s;
+        add(x, y, z);
+        add(x, y + hs, z + hs);
+        add(x + hs, y, z + hs);
+        add(x + hs, y + hs, z);
+      }
+  *box = L;
+  return LJGPU_OK;
+}
+
+// Cell-list Verlet build with the reference's exact pair set and CSR order
+// (cpu_ref/force_aos.cpp:100-140; cuda/force_cuda.cu:122-163).
+int ljgpu_makepair_host(int32_t n, const double* q, double box, double search_length, int kind,
+                        int32_t* nop, int32_t* pointer, int32_t* sorted_list, int64_t capacity,
+                        int64_t* n_entries) {
+  if (n < 0 || (n > 0 && (!q || !nop || !pointer)) || !n_entries || !(search_length > 0.0))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "makepair_host: bad argument");
+  if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "makepair_host: bad kind");
+  if (box > 0.0 && box < 2.0 * search_length)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "makepair_host: periodic box shorter than twice the search length");
+  const double sl2 = search_length * search_length;
+  *n_entries = 0;
+  if (n == 0) return LJGPU_OK;
+
+  // Cell grid: cells of side >= search_length; a periodic box needs at least
+  // three cells per side for the 27-cell stencil to visit each cell once,
+  // otherwise every particle is a candidate (brute force).
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; c++) {
+    lo[c] = hi[c] = q[c];
+  }
+  for (int32_t i = 0; i < n; i++)
+    for (int c = 0; c < 3; c++) {
+      lo[c] = std::min(lo[c], q[4 * (int64_t)i + c]);
+      hi[c] = std::max(hi[c], q[4 * (int64_t)i + c]);
+    }
+  int nc[3];
+  double origin[3], width[3];
+  bool brute = false;
+  for (int c = 0; c < 3; c++) {
+    const double extent = box > 0.0 ? box : (hi[c] - lo[c]);
+    origin[c] = box > 0.0 ? 0.0 : lo[c];
+    int m = static_cast<int>(extent / search_length);
+    if (m < 1) m = 1;
+    if (m > 1024) m = 1024;
+    nc[c] = m;
+    width[c] = extent > 0.0 ? extent / m : 1.0;
+    if (box > 0.0 && m < 3) brute = true;
+  }
+  const int64_t ncell = static_cast<int64_t>(nc[0]) * nc[1] * nc[2];
+  std::vector<int32_t> cell_of(n), cell_start(ncell + 1, 0), cell_items(n);
+  for (int32_t i = 0; i < n; i++) {
+    int ci[3];
+    for (int c = 0; c < 3; c++) {
+      double x = q[4 * (int64_t)i + c] - origin[c];
+      if (box > 0.0) x -= box * std::floor(x / box);  // cell of the wrapped image
+      int v = static_cast<int>(std::floor(x / width[c]));
+      ci[c] = std::min(std::max(v, 0), nc[c] - 1);
+    }
+    cell_of[i] = static_cast<int32_t>((static_cast<int64_t>(ci[2]) * nc[1] + ci[1]) * nc[0] + ci[0]);
+    cell_start[cell_of[i] + 1]++;
+  }
+  for (int64_t c = 0; c < ncell; c++) cell_start[c + 1] += cell_start[c];
+  {
+    std::vector<int32_t> fill(cell_start.begin(), cell_start.end() - 1);
+    for (int32_t i = 0; i < n; i++) cell_items[fill[cell_of[i]]++] = i;  // ascending i per cell
+  }
+
+  // Rows are built in parallel into per-row vectors, then concatenated in i
+  // order, so the CSR does not depend on the thread count.
+  std::vector<std::vector<int32_t>> rows(n);
+  auto build_rows = [&](int32_t i0, int32_t i1) {
+    std::vector<int32_t> cand;
+    for (int32_t i = i0; i < i1; i++) {
+      const double xi = q[4 * (int64_t)i], yi = q[4 * (int64_t)i + 1], zi = q[4 * (int64_t)i + 2];
+      std::vector<int32_t>& row = rows[i];
+      cand.clear();
+      if (brute) {
+        for (int32_t j = 0; j < n; j++) cand.push_back(j);
+      } else {
+        const int32_t c = cell_of[i];
+        const int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+        for (int dz = -1; dz <= 1; dz++) {
+          int z = cz + dz;
+          if (box > 0.0) z = (z + nc[2]) % nc[2];
+          else if (z < 0 || z >= nc[2]) continue;
+          for (int dy = -1; dy <= 1; dy++) {
+            int y = cy + dy;
+            if (box > 0.0) y = (y + nc[1]) % nc[1];
+            else if (y < 0 || y >= nc[1]) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+              int x = cx + dx;
+              if (box > 0.0) x = (x + nc[0]) % nc[0];
+              else if (x < 0 || x >= nc[0]) continue;
+              const int64_t cc = (static_cast<int64_t>(z) * nc[1] + y) * nc[0] + x;
+              for (int32_t t = cell_start[cc]; t < cell_start[cc + 1]; t++)
+                cand.push_back(cell_items[t]);
+            }
+          }
+        }
+      }
+      for (int32_t j : cand) {
+        if (kind == LJGPU_LIST_HALF ? (j <= i) : (j == i)) continue;
+        const double dx = min_image(xi - q[4 * (int64_t)j], box);
+        const double dy = min_image(yi - q[4 * (int64_t)j + 1], box);
+        const double dz = min_image(zi - q[4 * (int64_t)j + 2], box);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < sl2) row.push_back(j);
+      }
+      std::sort(row.begin(), row.end());
+    }
+  };
+  unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nthreads = 1;
+  std::vector<std::thread> pool;
+  const int32_t chunk = (n + nthreads - 1) / nthreads;
+  for (unsigned t = 0; t < nthreads; t++) {
+    const int32_t a = static_cast<int32_t>(std::min<int64_t>((int64_t)t * chunk, n));
+    const int32_t b = static_cast<int32_t>(std::min<int64_t>((int64_t)a + chunk, n));
+    if (a < b) pool.emplace_back(build_rows, a, b);
+  }
+  for (auto& th : pool) th.join();
+
+  int64_t pos = 0;
+  for (int32_t i = 0; i < n; i++) {
+    nop[i] = static_cast<int32_t>(rows[i].size());
+    pointer[i] = static_cast<int32_t>(std::min<int64_t>(pos, INT32_MAX));
+    if (sorted_list && pos + static_cast<int64_t>(rows[i].size()) <= capacity)
+      std::memcpy(sorted_list + pos, rows[i].data(), rows[i].size() * sizeof(int32_t));
+    pos += static_cast<int64_t>(rows[i].size());
+  }
+  *n_entries = pos;
+  if (pos > capacity || pos > INT32_MAX)
+    return set_error(LJGPU_ERR_CAPACITY, "makepair_host: list exceeds capacity (" +
+                                             std::to_string(pos) + " entries)");
+  return LJGPU_OK;
+}
+
+int ljgpu_validate_csr(int32_t n, const int32_t* nop, const int32_t* ptr,
+                       const int32_t* list, int64_t n_entries, int kind) {
+  return validate_csr(n, nop, ptr, list, n_entries, kind);
+}
+
+// cuda/force_cuda.cu:255-263.
+int ljgpu_random_shfl(int32_t n, const int32_t* nop, const int32_t* ptr, int32_t* list,
+                      uint32_t seed) {
+  if (n < 0 || (n > 0 && (!nop || !ptr || !list)))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "random_shfl: bad argument");
+  std::mt19937 mt(seed);
+  for (int32_t i = 0; i < n; i++) std::shuffle(list + ptr[i], list + ptr[i] + nop[i], mt);
+  return LJGPU_OK;
+}
+
+// cuda/force_cuda.cu:229-240.
+int ljgpu_make_transposed(int32_t n, const int32_t* nop, const int32_t* ptr,
+                          const int32_t* list, int32_t width, int32_t* out) {
+  if (n < 0 || width < 0 || (n > 0 && (!nop || !ptr || !out)))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_transposed: bad argument");
+  for (int32_t i = 0; i < n; i++)
+    if (nop[i] > width) return set_error(LJGPU_ERR_CAPACITY, "make_transposed: row longer than width");
+  std::memset(out, 0, sizeof(int32_t) * static_cast<size_t>(n) * width);
+  for (int32_t i = 0; i < n; i++)
+    for (int32_t k = 0; k < nop[i]; k++) out[i + static_cast<int64_t>(k) * n] = list[ptr[i] + k];
+  return LJGPU_OK;
+}
+
+}  // extern "C"

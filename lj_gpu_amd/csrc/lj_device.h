@@ -1,0 +1,161 @@
+// lj_device.h — device-side building blocks shared by the force kernels.
+//
+// Layouts: AoS positions/momenta are double[N][4] (x, y, z, pad; 32 B per
+// particle, the reference's double4), SoA three double[N] arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ljgpu {
+
+// Number of XCDs on MI355X. Workgroups are dispatched round-robin over them
+// (workgroup b runs on XCD b % 8), each XCD with its own L2.
+constexpr int kXcds = 8;
+
+// Logical block index for an XCD-aware launch over a grid padded to a
+// multiple of kXcds: XCD x receives the contiguous logical range
+// [x*per, (x+1)*per), so particles with consecutive indices (which share most
+// neighbours in lattice order) are served by the same L2.
+__device__ __forceinline__ int xcd_logical_block(int nblocks_padded) {
+  const int b = static_cast<int>(blockIdx.x);
+  const int per = nblocks_padded / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
+struct PosAos {
+  const double* __restrict__ q;
+  __device__ __forceinline__ void ld(int j, double& x, double& y, double& z) const {
+    const double* r = q + 4 * static_cast<int64_t>(j);
+    const double2 xy = *reinterpret_cast<const double2*>(r);
+    x = xy.x;
+    y = xy.y;
+    z = r[2];
+  }
+};
+
+struct PosSoa {
+  const double* __restrict__ x;
+  const double* __restrict__ y;
+  const double* __restrict__ z;
+  __device__ __forceinline__ void ld(int j, double& a, double& b, double& c) const {
+    a = x[j];
+    b = y[j];
+    c = z[j];
+  }
+};
+
+struct MomAos {
+  double* __restrict__ p;
+  __device__ __forceinline__ void add(int i, double ax, double ay, double az) const {
+    double* r = p + 4 * static_cast<int64_t>(i);
+    double2 xy = *reinterpret_cast<double2*>(r);
+    xy.x += ax;
+    xy.y += ay;
+    *reinterpret_cast<double2*>(r) = xy;
+    r[2] += az;
+  }
+  __device__ __forceinline__ void atomic_add(int i, double ax, double ay, double az) const {
+    double* r = p + 4 * static_cast<int64_t>(i);
+    unsafeAtomicAdd(r, ax);
+    unsafeAtomicAdd(r + 1, ay);
+    unsafeAtomicAdd(r + 2, az);
+  }
+};
+
+struct MomSoa {
+  double* __restrict__ x;
+  double* __restrict__ y;
+  double* __restrict__ z;
+  __device__ __forceinline__ void add(int i, double ax, double ay, double az) const {
+    x[i] += ax;
+    y[i] += ay;
+    z[i] += az;
+  }
+  __device__ __forceinline__ void atomic_add(int i, double ax, double ay, double az) const {
+    unsafeAtomicAdd(x + i, ax);
+    unsafeAtomicAdd(y + i, ay);
+    unsafeAtomicAdd(z + i, az);
+  }
+};
+
+// Pair constants: c24 = 24*dt and c48 = 48*dt fold the time step into the
+// force scalar; cl2 is the squared cutoff (pairs with r2 > cl2 contribute
+// nothing, r2 == cl2 is kept, as in the reference's `if (r2 > CL2)`).
+struct PairParams {
+  double c24, c48, cl2, box, inv_box;
+  float c24f, c48f;
+};
+
+// Minimum image in a periodic box: d - box * rint(d / box). For positions
+// inside [0, box) the shift is -1, 0 or +1 box lengths and the result is the
+// same double as the oracle's threshold form (d > box/2 ? d - box : ...).
+template <bool PBC>
+__device__ __forceinline__ void min_image(double& dx, double& dy, double& dz,
+                                          const PairParams& pp) {
+  if (PBC) {
+    dx = fma(-pp.box, rint(dx * pp.inv_box), dx);
+    dy = fma(-pp.box, rint(dy * pp.inv_box), dy);
+    dz = fma(-pp.box, rint(dz * pp.inv_box), dz);
+  }
+}
+
+// 1/x in FP64: the hardware reciprocal estimate refined by two
+// Newton-Raphson steps r <- r + r*(1 - x*r) (quadratic convergence; the
+// refined value is within a few ulp of the correctly rounded quotient).
+// Valid for the finite, normal x of a pair distance.
+__device__ __forceinline__ double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
+// FP64 force scalar times dt: ((24 r^6 - 48) / r^14) * dt rewritten with
+// s = 1/r2 as s^4 * (24 - 48 s^3) * dt (one reciprocal, no divide).
+__device__ __forceinline__ double lj_df_f64(double r2, const PairParams& pp) {
+  const double s = rcp_f64(r2);
+  const double s3 = s * s * s;
+  const double df = (s3 * s) * fma(-pp.c48, s3, pp.c24);
+  return (r2 > pp.cl2) ? 0.0 : df;
+}
+
+// Mixed-precision force scalar: r2 (computed in FP64) rounded once to FP32,
+// then the reference's form (24 r^6 - 48) / (r^6 r^6 r^2) in FP32 with the
+// FP32 reciprocal refined by one Newton-Raphson step. The cutoff test uses
+// the FP64 r2 so the pair set is the same as in the FP64 kernels.
+__device__ __forceinline__ float lj_df_mixed(double r2, const PairParams& pp) {
+  const float r2f = static_cast<float>(r2);
+  const float r6 = r2f * r2f * r2f;
+  const float r14 = r6 * r6 * r2f;
+  float inv = __builtin_amdgcn_rcpf(r14);
+  inv = fmaf(fmaf(-r14, inv, 1.0f), inv, inv);
+  const float df = fmaf(pp.c24f, r6, -pp.c48f) * inv;
+  return (r2 > pp.cl2) ? 0.0f : df;
+}
+
+// Kahan-compensated FP32 running sum.
+struct KahanF32 {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const {
+    return static_cast<double>(s) - static_cast<double>(c);
+  }
+};
+
+// Sum over the `W` consecutive lanes of a wavefront segment (W a power of two
+// <= 64); every lane of the segment receives the total.
+template <int W>
+__device__ __forceinline__ double segment_sum(double v) {
+#pragma unroll
+  for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
+  return v;
+}
+
+}  // namespace ljgpu

@@ -1,0 +1,79 @@
+"""Shared workloads and oracle results for the test suite (cached per process)."""
+from __future__ import annotations
+
+import functools
+import os
+
+import numpy as np
+
+import lj_gpu_amd as G
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_DATA_05 = os.path.join(GOLDEN, "ref_data_density0.5.dat")
+REF_DATA_1 = os.path.join(GOLDEN, "ref_data_density1.dat")
+
+# Periodic workloads: FCC 4*k^3 particles, seeded jitter U(-0.05, 0.05),
+# cutoff 3.0, search length 3.3, dt = 0.01, 100 steps.
+PBC_DT = 0.01
+STEPS = 100
+FP64_TOL = 1e-12  # normwise max|dp| / max|p| against the FP64 oracle
+
+
+def normwise(a: np.ndarray, ref: np.ndarray) -> float:
+    a = np.asarray(a)[..., :3]
+    ref = np.asarray(ref)[..., :3]
+    scale = np.abs(ref).max()
+    return float(np.abs(a - ref).max() / scale) if scale > 0 else float(np.abs(a).max())
+
+
+def rms_normwise(a: np.ndarray, ref: np.ndarray) -> float:
+    a = np.asarray(a)[..., :3]
+    ref = np.asarray(ref)[..., :3]
+    return float(np.sqrt(((a - ref) ** 2).mean()) / np.abs(ref).max())
+
+
+def load_golden(name: str) -> dict:
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@functools.lru_cache(maxsize=None)
+def periodic(k: int, density: float, seed: int = 2):
+    """(q (N,4), box, half CSR, full CSR) from the product's host builder."""
+    q, box = G.lattice_periodic(k, density, seed)
+    half = G.makepair(q, box, G.LIST_HALF)
+    full = G.makepair(q, box, G.LIST_FULL)
+    return q, box, half, full
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_momenta(k: int, density: float, steps: int = STEPS, dt: float = PBC_DT):
+    """Oracle force_sorted (half list, j scatter) after `steps` steps from p = 0."""
+    q, box, half, _ = periodic(k, density)
+    csr = O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half")
+    p = np.zeros_like(q)
+    O.force_aos(csr, q, p, dt, O.CL2, box, steps=steps, method="sorted")
+    return p
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_force(k: int, density: float):
+    """Oracle F (one step with dt = 1) from the full list."""
+    q, box, _, full = periodic(k, density)
+    csr = O.CSR(full.number_of_partners, full.pointer, full.sorted_list, "full")
+    p = np.zeros_like(q)
+    O.force_aos(csr, q, p, 1.0, O.CL2, box, steps=1, method="full")
+    return p
+
+
+@functools.lru_cache(maxsize=None)
+def reference_mode(density: float = 0.5):
+    """The reference's own problem: open box L=50, U(0,0.1) jitter, half list."""
+    q = G.lattice_reference(density, 50.0)
+    half = G.makepair(q, 0.0, G.LIST_HALF)
+    return q, half
+
+
+def to_soa(q4: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(q4[:, :3].T)
