@@ -78,6 +78,7 @@ def main() -> int:
     ap.add_argument("--density", type=float, default=1.0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
+    ap.add_argument("--unroll", type=int, default=0, help="entries per lane per batch (0: default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
@@ -104,7 +105,7 @@ def main() -> int:
     log(f"setup: N={n} box={box:.4f} full entries={e} pairs={pairs} ({time.time() - t0:.1f}s)")
 
     ctx = G.Context(0)
-    ctx.set_kernel_config(args.lanes, args.block)
+    ctx.set_kernel_config(args.lanes, args.block, args.unroll)
     ctx.set_list(full)
     info = ctx.list_info()
     dq = ctx.array(q)

@@ -79,6 +79,11 @@ typedef struct ljgpu_list_info {
   int32_t min_partners;    /* shortest row                                  */
   int32_t half_mode;       /* LJGPU_HALF_ATOMIC / LJGPU_HALF_EXPAND in use  */
   int32_t lanes_per_particle; /* kernel configuration in use                */
+  int32_t unroll;
+  int32_t block_size;
+  int32_t tile_particles;  /* LDS tiling in use: particles per block (0: off) */
+  int32_t tile_blocks;     /* number of LDS tiles                            */
+  int32_t tile_max_stage;  /* most positions one tile stages in LDS          */
 } ljgpu_list_info;
 
 /* ---- context, errors, device memory ----------------------------------- */
@@ -141,10 +146,24 @@ int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode);
 
 /* Kernel configuration: lanes cooperating on one i-particle (1 selects the
  * thread-per-particle ELL kernel; 4, 8, 16, 32 or 64 the CSR
- * wavefront-segment kernel) and the workgroup size (64..1024, multiple of
- * 64). 0 keeps the current value. The reference exposes only the block size
- * (argv[1], cuda/force_cuda.cu:377-383). */
-int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_size);
+ * wavefront-segment kernel), the workgroup size (64..1024, multiple of 64)
+ * and the list entries each lane keeps in flight per pipelined batch
+ * (unroll 1, 2 or 4). 0 keeps the current value. The reference exposes only
+ * the block size (argv[1], cuda/force_cuda.cu:377-383). */
+int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_size, int unroll);
+
+/* LDS tiling of full lists (on by default, 64 particles per tile): the list
+ * is regrouped once per ljgpu_set_list_* into blocks of consecutive
+ * particles whose partners' positions are staged in LDS, and each entry is
+ * re-expressed as a 16-bit LDS offset. 0 turns tiling off (global-gather
+ * kernels). The tile actually used is capped at 1024 / lanes_per_particle
+ * particles (one workgroup per tile); ELL lists and half lists evaluated
+ * with atomics are never tiled. stage_offsets = 1 also stages each tile's
+ * entry offsets in LDS (more LDS per workgroup); 0 streams them from HBM.
+ * No reference counterpart: it plays the role of the reference's derived
+ * transposed list (cuda/force_cuda.cu:229-240), a layout built once from the
+ * CSR for the device. */
+int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets);
 
 /* ---- forces ------------------------------------------------------------ */
 /* p += F(q) * dt over the installed list, repeated `steps` times (the
@@ -224,6 +243,21 @@ int ljgpu_random_shfl(int32_t particle_number, const int32_t* number_of_partners
 int ljgpu_make_transposed(int32_t particle_number, const int32_t* number_of_partners,
                           const int32_t* pointer, const int32_t* sorted_list, int32_t width,
                           int32_t* transposed_list);
+
+/* The LDS tiling ljgpu_set_tiling installs, built on the host from a CSR
+ * list whose rows are stored in order (pointer[i+1] = pointer[i] +
+ * number_of_partners[i], as sortpair produces): the tiled counterpart of
+ * make_transposed_pairlist, cuda/force_cuda.cu:229-240. desc receives 8 ints
+ * per block (first particle, particle count, staging offset, staging count,
+ * first list entry, entry count, 0, 0), stage the staged particle indices,
+ * local (n_entries values, parallel to sorted_list) the LDS byte offset
+ * 24*slot of each entry's j within its block's staging area; max_stage the
+ * largest staging count. Pass null outputs to query the sizes first. */
+int ljgpu_make_tiles(int32_t particle_number, const int32_t* number_of_partners,
+                     const int32_t* pointer, const int32_t* sorted_list, int64_t n_entries,
+                     int32_t tile_particles, int32_t* desc, int64_t desc_capacity,
+                     int32_t* stage, int64_t stage_capacity, uint16_t* local,
+                     int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage);
 
 #ifdef __cplusplus
 }

@@ -67,7 +67,10 @@ class LJGPUError(RuntimeError):
 class ListInfo(C.Structure):
     _fields_ = [("particle_number", C.c_int32), ("kind", C.c_int32), ("n_entries", C.c_int64),
                 ("n_pairs", C.c_int64), ("max_partners", C.c_int32), ("min_partners", C.c_int32),
-                ("half_mode", C.c_int32), ("lanes_per_particle", C.c_int32)]
+                ("half_mode", C.c_int32), ("lanes_per_particle", C.c_int32),
+                ("unroll", C.c_int32), ("block_size", C.c_int32),
+                ("tile_particles", C.c_int32), ("tile_blocks", C.c_int32),
+                ("tile_max_stage", C.c_int32)]
 
 
 _lib = None
@@ -103,7 +106,8 @@ def lib() -> C.CDLL:
         "ljgpu_get_list_info": [_vp, C.POINTER(ListInfo)],
         "ljgpu_get_list_csr": [_vp, _vp, _vp, _vp, _i64],
         "ljgpu_set_half_mode": [_vp, C.c_int],
-        "ljgpu_set_kernel_config": [_vp, C.c_int, C.c_int],
+        "ljgpu_set_tiling": [_vp, _i32, C.c_int],
+        "ljgpu_set_kernel_config": [_vp, C.c_int, C.c_int, C.c_int],
         "ljgpu_force_aos_f64": [_vp, _vp, _vp, _d, _d, _d, _i32],
         "ljgpu_force_soa_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32],
         "ljgpu_force_aos_mixed": [_vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
@@ -115,6 +119,8 @@ def lib() -> C.CDLL:
         "ljgpu_validate_csr": [_i32, _vp, _vp, _vp, _i64, C.c_int],
         "ljgpu_random_shfl": [_i32, _vp, _vp, _vp, C.c_uint32],
         "ljgpu_make_transposed": [_i32, _vp, _vp, _vp, _i32, _vp],
+        "ljgpu_make_tiles": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64, _vp,
+                             C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -233,6 +239,31 @@ def make_transposed(csr: CSR, width: int | None = None) -> np.ndarray:
     return out[: csr.n * width]
 
 
+@dataclass
+class Tiles:
+    """LDS tiling of a full list (what ljgpu_set_tiling installs)."""
+    desc: np.ndarray      # (n_blocks, 8): first particle, count, staging offset,
+                          # staging count, first entry, entry count, 0, 0
+    stage: np.ndarray     # staged particle indices
+    local: np.ndarray     # uint16 LDS byte offsets, parallel to sorted_list
+    max_stage: int
+
+
+def make_tiles(csr: CSR, tile_particles: int = 64) -> Tiles:
+    nb, ns, ms = _i32(0), _i64(0), _i32(0)
+    args = (csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer), _ptr(csr.sorted_list),
+            csr.n_entries, tile_particles)
+    _check(lib().ljgpu_make_tiles(*args, None, 0, None, 0, None, C.byref(nb), C.byref(ns),
+                                  C.byref(ms)))
+    desc = np.zeros(max(8 * nb.value, 1), np.int32)
+    stage = np.zeros(max(ns.value, 1), np.int32)
+    local = np.zeros(max(csr.n_entries, 1), np.uint16)
+    _check(lib().ljgpu_make_tiles(*args, _ptr(desc), desc.size, _ptr(stage), stage.size,
+                                  _ptr(local), C.byref(nb), C.byref(ns), C.byref(ms)))
+    return Tiles(desc[: 8 * nb.value].reshape(-1, 8), stage[: ns.value],
+                 local[: csr.n_entries], ms.value)
+
+
 # ---------------------------------------------------------------------------
 # Device side
 # ---------------------------------------------------------------------------
@@ -315,11 +346,18 @@ class Context:
         _check(lib().ljgpu_sync(self.handle))
 
     # configuration
-    def set_kernel_config(self, lanes: int = 0, block: int = 0) -> None:
-        _check(lib().ljgpu_set_kernel_config(self.handle, lanes, block))
+    def set_kernel_config(self, lanes: int = 0, block: int = 0, unroll: int = 0) -> None:
+        """Lanes per i-particle (1: ELL kernel), workgroup size, entries per
+        lane per pipelined batch; 0 keeps the current value."""
+        _check(lib().ljgpu_set_kernel_config(self.handle, lanes, block, unroll))
 
     def set_half_mode(self, mode: int) -> None:
         _check(lib().ljgpu_set_half_mode(self.handle, mode))
+
+    def set_tiling(self, tile_particles: int, stage_offsets: bool = False) -> None:
+        """Particles per LDS tile (0: untiled global-gather kernels); with
+        stage_offsets the tile's entry offsets are staged in LDS as well."""
+        _check(lib().ljgpu_set_tiling(self.handle, tile_particles, int(stage_offsets)))
 
     # lists
     def set_list(self, csr: CSR) -> None:

@@ -36,7 +36,17 @@ struct ljgpu_ctx {
   int32_t* d_ell = nullptr;
   int32_t ell_width = 0;
 
-  int lanes = 16;
+  // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
+  int32_t tile_particles = 64;
+  bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
+  int32_t tiles_built_for = 0;  // tile_particles the device tiles were built with
+  HostTiles tiles;
+  int32_t* d_tile_desc = nullptr;
+  int32_t* d_tile_stage = nullptr;
+  uint16_t* d_tile_local = nullptr;
+
+  int lanes = 8;
+  int unroll = 2;
   int block = 256;
 };
 
@@ -58,7 +68,18 @@ int use_device(ljgpu_ctx* ctx) {
   return LJGPU_OK;
 }
 
+void free_tiles(ljgpu_ctx* ctx) {
+  if (ctx->d_tile_desc) (void)hipFree(ctx->d_tile_desc);
+  if (ctx->d_tile_stage) (void)hipFree(ctx->d_tile_stage);
+  if (ctx->d_tile_local) (void)hipFree(ctx->d_tile_local);
+  ctx->d_tile_desc = ctx->d_tile_stage = nullptr;
+  ctx->d_tile_local = nullptr;
+  ctx->tiles = HostTiles();
+  ctx->tiles_built_for = 0;
+}
+
 void free_list(ljgpu_ctx* ctx) {
+  free_tiles(ctx);
   if (ctx->d_nop) (void)hipFree(ctx->d_nop);
   if (ctx->d_ptr) (void)hipFree(ctx->d_ptr);
   if (ctx->d_list) (void)hipFree(ctx->d_list);
@@ -100,6 +121,38 @@ int build_ell(ljgpu_ctx* ctx) {
   return LJGPU_OK;
 }
 
+// Derive and upload the LDS tiling of the evaluated (full) list for the
+// context's tile_particles. A list whose single rows exceed the LDS staging
+// capacity keeps the untiled kernels.
+// Particles per tile for the current kernel configuration: the requested
+// tile size, capped so that a tile's workgroup (tile * lanes threads) fits
+// 1024 threads; 0 when tiling is off or the ELL kernel is selected.
+int32_t effective_tile(const ljgpu_ctx* ctx) {
+  if (ctx->tile_particles <= 0 || ctx->lanes <= 1) return 0;
+  return std::min(ctx->tile_particles, 1024 / ctx->lanes);
+}
+
+int build_device_tiles(ljgpu_ctx* ctx) {
+  free_tiles(ctx);
+  const int32_t tp = effective_tile(ctx);
+  if (tp <= 0 || ctx->eval_half || ctx->eval.n == 0) return LJGPU_OK;
+  HostTiles t;
+  if (!build_tiles(ctx->eval, tp, &t)) return LJGPU_OK;
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_tile_desc, t.desc))) return rc;
+  if ((rc = upload(ctx, &ctx->d_tile_stage, t.stage))) return rc;
+  t.local.resize(t.local.size() + kTileLocalPad, 0);  // the kernel copies whole 16-byte chunks
+  if ((rc = upload(ctx, &ctx->d_tile_local, t.local))) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  ctx->tiles = std::move(t);
+  ctx->tiles.stage.clear();
+  ctx->tiles.stage.shrink_to_fit();
+  ctx->tiles.local.clear();
+  ctx->tiles.local.shrink_to_fit();
+  ctx->tiles_built_for = tp;
+  return LJGPU_OK;
+}
+
 // Install a validated CSR list held on the host.
 int install_csr(ljgpu_ctx* ctx, int32_t n, std::vector<int32_t>&& nop, std::vector<int32_t>&& ptr,
                 std::vector<int32_t>&& list, int kind) {
@@ -133,6 +186,8 @@ int install_csr(ljgpu_ctx* ctx, int32_t n, std::vector<int32_t>&& nop, std::vect
     ctx->eval = std::move(given);
     ctx->eval_half = (kind == LJGPU_LIST_HALF);
   }
+  // Rows stored in order make every tile's entries one contiguous range.
+  csr_make_row_ordered(&ctx->eval);
   if ((rc = upload(ctx, &ctx->d_nop, ctx->eval.nop))) return rc;
   if ((rc = upload(ctx, &ctx->d_ptr, ctx->eval.ptr))) return rc;
   if ((rc = upload(ctx, &ctx->d_list, ctx->eval.list))) return rc;
@@ -140,6 +195,8 @@ int install_csr(ljgpu_ctx* ctx, int32_t n, std::vector<int32_t>&& nop, std::vect
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   if (ctx->lanes == 1) {
     if ((rc = build_ell(ctx))) return rc;
+  } else if (effective_tile(ctx) > 0) {
+    if ((rc = build_device_tiles(ctx))) return rc;
   }
   return LJGPU_OK;
 }
@@ -166,7 +223,9 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
   lv.n_entries = ctx->d_list_len;
   LaunchConfig lc;
   lc.lanes = ctx->lanes;
+  lc.unroll = ctx->unroll;
   lc.block = ctx->block;
+  lc.tile_offsets_lds = ctx->tile_offsets_lds;
   lc.half_atomic = ctx->eval_half;
   lc.clamp = false;
   if (ctx->lanes == 1) {
@@ -175,10 +234,24 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
     lv.ell_width = ctx->ell_width;
   } else {
     lv.list = ctx->d_list;
+    const int32_t tp = effective_tile(ctx);
+    if (tp > 0 && !ctx->eval_half) {
+      if (ctx->tiles_built_for != tp && (rc = build_device_tiles(ctx))) return rc;
+      if (ctx->tiles_built_for > 0) {
+        lv.tile_blocks = static_cast<int32_t>(ctx->tiles.desc.size() / 8);
+        lv.tile_particles = ctx->tiles.tile_particles;
+        lv.tile_lds_bytes = ctx->tiles.lds_bytes;
+        lv.tile_max_stage = ctx->tiles.max_stage;
+        lv.tile_desc = ctx->d_tile_desc;
+        lv.tile_stage = ctx->d_tile_stage;
+        lv.tile_local = ctx->d_tile_local;
+      }
+    }
   }
   for (int32_t s = 0; s < steps; s++) {
-    rc = mixed ? launch_force_mixed(ctx->stream, lv, fa, lc, accum)
-               : launch_force_f64(ctx->stream, lv, fa, lc);
+    rc = !mixed ? launch_force_f64(ctx->stream, lv, fa, lc)
+         : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(ctx->stream, lv, fa, lc)
+                                          : launch_force_mixed_f64acc(ctx->stream, lv, fa, lc);
     if (rc) return rc;
   }
   return LJGPU_OK;
@@ -313,6 +386,17 @@ int ljgpu_timer_stop(ljgpu_ctx* ctx, float* ms) {
   return LJGPU_OK;
 }
 
+int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (tile_particles < 0 || tile_particles > 1024)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "tile_particles must be in [0, 1024]");
+  if (stage_offsets != 0 && stage_offsets != 1)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "stage_offsets must be 0 or 1");
+  ctx->tile_particles = tile_particles;
+  ctx->tile_offsets_lds = stage_offsets != 0;
+  return LJGPU_OK;
+}
+
 int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (half_mode != LJGPU_HALF_AUTO && half_mode != LJGPU_HALF_ATOMIC && half_mode != LJGPU_HALF_EXPAND)
@@ -321,14 +405,17 @@ int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode) {
   return LJGPU_OK;
 }
 
-int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes, int block) {
+int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes, int block, int unroll) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "lanes_per_particle must be 1,4,8,16,32 or 64");
   if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "block size must be a multiple of 64 in [64, 1024]");
+  if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
   if (lanes) ctx->lanes = lanes;
   if (block) ctx->block = block;
+  if (unroll) ctx->unroll = unroll;
   return LJGPU_OK;
 }
 
@@ -410,6 +497,11 @@ int ljgpu_get_list_info(ljgpu_ctx* ctx, ljgpu_list_info* info) {
   info->min_partners = ctx->min_np;
   info->half_mode = ctx->half_mode;
   info->lanes_per_particle = ctx->lanes;
+  info->unroll = ctx->unroll;
+  info->block_size = ctx->block;
+  info->tile_particles = ctx->tiles_built_for;
+  info->tile_blocks = static_cast<int32_t>(ctx->tiles.desc.size() / 8);
+  info->tile_max_stage = ctx->tiles.max_stage;
   return LJGPU_OK;
 }
 
@@ -529,6 +621,7 @@ int ljgpu_force_kernel_csr_f64(ljgpu_ctx* ctx, const double* q, double* p, int32
   fa.box = 0.0;
   LaunchConfig lc;
   lc.lanes = ctx->lanes == 1 ? 16 : ctx->lanes;
+  lc.unroll = ctx->unroll;
   lc.block = ctx->block;
   lc.half_atomic = (kind == LJGPU_LIST_HALF);
   lc.clamp = true;

@@ -1,0 +1,577 @@
+// force_kernels.h — Lennard-Jones force kernels for gfx950 (MI355X), templated
+// over the precision policy; each policy is instantiated in its own
+// translation unit (force_f64.hip, force_mixed.hip, force_kahan.hip).
+//
+// p += F(q) * dt over a Verlet list, the pair term of cpu_ref/force_aos.cpp:176-189
+// and cuda/kernel.cuh:22-31. Two kernel families:
+//
+//  * k_csr: a segment of TPI consecutive lanes of a 64-wide wavefront owns one
+//    i-particle and walks its CSR row in batches of U*TPI entries: lane t of
+//    the segment takes entries base + u*TPI + t (u < U), so each batch reads
+//    U contiguous TPI-wide chunks of the row. All U gathers of a batch are
+//    issued before any is used, and the indices of the next batch are loaded
+//    while the current batch is computed (a two-stage software pipeline), so
+//    a wavefront keeps 2U-3U memory operations in flight instead of one
+//    dependent index->gather chain per entry. The per-lane partial forces
+//    are summed across the segment with cross-lane shuffles and one lane
+//    updates p[i]. TPI = 64 is the wavefront-per-i form of the reference's
+//    warp_unroll kernels (cuda/kernel.cuh:821-904, 32-lane warps); smaller
+//    TPI packs several rows of ~80-150 entries into one wavefront so fewer
+//    lanes idle on the last batch of a row.
+//  * k_ell: one lane per i-particle over the reference's transposed (ELL)
+//    list, entry k of row i at list[i + k*N] so a wavefront reads 64
+//    consecutive indices per entry (force_kernel_memopt2 and relatives,
+//    cuda/kernel.cuh:102-236), with the same U-batch pipeline.
+//
+// Both take full lists (each row updates only its own i, like the
+// reference's full-list kernels) or half lists (HALF: every pair is visited
+// once and -f is scattered to j with hardware FP64 atomics, like the
+// *_with_aar kernels, cuda/kernel.cuh:238-469).
+//
+// Precision policies (PREC): kF64 — FP64 throughout; kMixedF64 — force scalar
+// in FP32, accumulation in FP64; kMixedKahan — force scalar and per-lane sums
+// in FP32 with Kahan compensation. Distances and r2 are FP64 in every policy.
+//
+// Launches are XCD-aware: the grid is padded to a multiple of 8 and block b
+// maps to a contiguous logical range per XCD (xcd_logical_block).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "lj_device.h"
+#include "ljgpu_internal.h"
+
+namespace ljgpu {
+namespace detail {
+
+constexpr int kF64 = 0;
+constexpr int kMixedF64 = 1;
+constexpr int kMixedKahan = 2;
+
+// Per-lane accumulator of the i-particle's force for each precision policy.
+template <int PREC>
+struct Accum {
+  double x = 0.0, y = 0.0, z = 0.0;
+  __device__ __forceinline__ void add(double df, float /*dff*/, double dx, double dy, double dz) {
+    x = fma(df, dx, x);
+    y = fma(df, dy, y);
+    z = fma(df, dz, z);
+  }
+  __device__ __forceinline__ double vx() const { return x; }
+  __device__ __forceinline__ double vy() const { return y; }
+  __device__ __forceinline__ double vz() const { return z; }
+};
+
+template <>
+struct Accum<kMixedKahan> {
+  KahanF32 x, y, z;
+  __device__ __forceinline__ void add(double /*df*/, float dff, double dx, double dy, double dz) {
+    x.add(dff * static_cast<float>(dx));
+    y.add(dff * static_cast<float>(dy));
+    z.add(dff * static_cast<float>(dz));
+  }
+  __device__ __forceinline__ double vx() const { return x.value(); }
+  __device__ __forceinline__ double vy() const { return y.value(); }
+  __device__ __forceinline__ double vz() const { return z.value(); }
+};
+
+// One list entry once q_j is in registers: minimum image, cutoff, force
+// scalar, accumulation into the i-accumulator and, for half lists, the
+// scatter of -f to j. `valid` is false for the padding slots of a batch.
+template <int PREC, bool PBC, bool HALF, class M>
+__device__ __forceinline__ void pair_compute(const M& mom, int j, bool valid, double dx,
+                                             double dy, double dz, const PairParams& pp,
+                                             Accum<PREC>& acc) {
+  min_image<PBC>(dx, dy, dz, pp);
+  const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+  double df;
+  float dff = 0.0f;
+  if constexpr (PREC == kF64) {
+    df = lj_df_f64(r2, pp);
+    df = valid ? df : 0.0;
+  } else {
+    dff = lj_df_mixed(r2, pp);
+    dff = valid ? dff : 0.0f;
+    df = static_cast<double>(dff);
+  }
+  acc.add(df, dff, dx, dy, dz);
+  if constexpr (HALF) {
+    if (valid && r2 <= pp.cl2) mom.atomic_add(j, -df * dx, -df * dy, -df * dz);
+  }
+}
+
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool HALF, bool CLAMP>
+__global__ void __launch_bounds__(1024)
+k_csr(P pos, M mom, int32_t n, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
+      const int32_t* __restrict__ list, int64_t n_entries, PairParams pp, int nblocks_padded) {
+  const int lb = xcd_logical_block(nblocks_padded);
+  const int64_t gt = static_cast<int64_t>(lb) * blockDim.x + threadIdx.x;
+  const int64_t i64 = gt / TPI;
+  const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
+  const bool active = i64 < n;
+  const int i = active ? static_cast<int>(i64) : 0;
+  Accum<PREC> acc;
+  if (active) {
+    double xi, yi, zi;
+    pos.ld(i, xi, yi, zi);
+    int np = nop[i];
+    int64_t kp = ptr[i];
+    if constexpr (CLAMP) {
+      kp = kp < 0 ? 0 : (kp > n_entries ? n_entries : kp);
+      const int64_t room = n_entries - kp;
+      np = np < 0 ? 0 : (np > room ? static_cast<int>(room) : np);
+    }
+    const int32_t* row = list + kp;
+    constexpr int kBatch = U * TPI;
+    // Indices of the first batch; -1 marks a padding slot past the row end.
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = u * TPI + sub;
+      jn[u] = k < np ? row[k] : -1;
+    }
+    for (int base = 0; base < np; base += kBatch) {
+      int j[U];
+      double xj[U], yj[U], zj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        j[u] = jn[u];
+        if constexpr (CLAMP) {
+          if (j[u] >= 0) j[u] = static_cast<int>(min(static_cast<uint32_t>(j[u]), static_cast<uint32_t>(n - 1)));
+        }
+        pos.ld(j[u] >= 0 ? j[u] : i, xj[u], yj[u], zj[u]);
+      }
+      const int next = base + kBatch;
+      if (next < np) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int k = next + u * TPI + sub;
+          jn[u] = k < np ? row[k] : -1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        pair_compute<PREC, PBC, HALF>(mom, j[u], j[u] >= 0, xj[u] - xi, yj[u] - yi, zj[u] - zi,
+                                      pp, acc);
+      }
+    }
+  }
+  const double fx = segment_sum<TPI>(acc.vx());
+  const double fy = segment_sum<TPI>(acc.vy());
+  const double fz = segment_sum<TPI>(acc.vz());
+  if (active && sub == 0) {
+    if constexpr (HALF) mom.atomic_add(i, fx, fy, fz);
+    else mom.add(i, fx, fy, fz);
+  }
+}
+
+template <int U, int PREC, class P, class M, bool PBC, bool HALF>
+__global__ void __launch_bounds__(1024)
+k_ell(P pos, M mom, int32_t n, const int32_t* __restrict__ nop, const int32_t* __restrict__ ell,
+      int32_t width, PairParams pp, int nblocks_padded) {
+  const int lb = xcd_logical_block(nblocks_padded);
+  const int64_t i64 = static_cast<int64_t>(lb) * blockDim.x + threadIdx.x;
+  if (i64 >= n) return;
+  const int i = static_cast<int>(i64);
+  double xi, yi, zi;
+  pos.ld(i, xi, yi, zi);
+  int np = nop[i];
+  np = np < 0 ? 0 : (np > width ? width : np);
+  Accum<PREC> acc;
+  const int32_t* col = ell + i;
+  int jn[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) jn[u] = u < np ? col[static_cast<int64_t>(u) * n] : -1;
+  for (int base = 0; base < np; base += U) {
+    int j[U];
+    double xj[U], yj[U], zj[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      j[u] = jn[u];
+      pos.ld(j[u] >= 0 ? j[u] : i, xj[u], yj[u], zj[u]);
+    }
+    if (base + U < np) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = base + U + u;
+        jn[u] = k < np ? col[static_cast<int64_t>(k) * n] : -1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      pair_compute<PREC, PBC, HALF>(mom, j[u], j[u] >= 0, xj[u] - xi, yj[u] - yi, zj[u] - zi, pp,
+                                    acc);
+  }
+  if constexpr (HALF) mom.atomic_add(i, acc.vx(), acc.vy(), acc.vz());
+  else mom.add(i, acc.vx(), acc.vy(), acc.vz());
+}
+
+// Shift a position by whole box lengths to the periodic image nearest the
+// reference point c (no-op in an open box).
+template <bool PBC>
+__device__ __forceinline__ void unwrap_near(double& x, double& y, double& z, double cx,
+                                            double cy, double cz, const PairParams& pp) {
+  if constexpr (PBC) {
+    x = fma(-pp.box, rint((x - cx) * pp.inv_box), x);
+    y = fma(-pp.box, rint((y - cy) * pp.inv_box), y);
+    z = fma(-pp.box, rint((z - cz) * pp.inv_box), z);
+  }
+}
+
+// One row of an LDS tile: entries [0, np) of the row, lane `sub` of the
+// segment taking entries base + u*TPI + sub. The row's entry offsets
+// (`lrow`) and the partner positions (`lb`) are both in LDS. Full batches of
+// U*TPI entries run unguarded; the remainder is walked one entry per lane at
+// a time (exec-masked, so a short tail costs at most one pass per TPI
+// entries). MI applies the minimum image per entry (blocks whose unwrapped
+// frame is not exact).
+template <int TPI, int U, int PREC, bool MI, class M>
+__device__ __forceinline__ void tile_row(const M& mom, const char* lb, const uint16_t* lrow,
+                                         int np, int sub, double xi, double yi, double zi,
+                                         const PairParams& pp, Accum<PREC>& acc) {
+  constexpr int kB = U * TPI;
+  int base = 0;
+  for (; base + kB <= np; base += kB) {
+    uint32_t off[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) off[u] = lrow[base + u * TPI + sub];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double xj = *reinterpret_cast<const double*>(lb + off[u]);
+      const double yj = *reinterpret_cast<const double*>(lb + off[u] + 8);
+      const double zj = *reinterpret_cast<const double*>(lb + off[u] + 16);
+      pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+    }
+  }
+  for (int k = base + sub; k < np; k += TPI) {
+    const uint32_t o = lrow[k];
+    const double xj = *reinterpret_cast<const double*>(lb + o);
+    const double yj = *reinterpret_cast<const double*>(lb + o + 8);
+    const double zj = *reinterpret_cast<const double*>(lb + o + 16);
+    pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+  }
+}
+
+// As tile_row, with the row's entry offsets read from global memory
+// (`lrow` in HBM): the offsets of the next full batch are loaded while the
+// current batch is computed.
+template <int TPI, int U, int PREC, bool MI, class M>
+__device__ __forceinline__ void tile_row_stream(const M& mom, const char* lb,
+                                                const uint16_t* __restrict__ lrow, int np,
+                                                int sub, double xi, double yi, double zi,
+                                                const PairParams& pp, Accum<PREC>& acc) {
+  constexpr int kB = U * TPI;
+  uint32_t on[U];
+  if (kB <= np) {
+#pragma unroll
+    for (int u = 0; u < U; u++) on[u] = lrow[u * TPI + sub];
+  }
+  int base = 0;
+  for (; base + kB <= np; base += kB) {
+    uint32_t off[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) off[u] = on[u];
+    if (base + 2 * kB <= np) {
+#pragma unroll
+      for (int u = 0; u < U; u++) on[u] = lrow[base + kB + u * TPI + sub];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double xj = *reinterpret_cast<const double*>(lb + off[u]);
+      const double yj = *reinterpret_cast<const double*>(lb + off[u] + 8);
+      const double zj = *reinterpret_cast<const double*>(lb + off[u] + 16);
+      pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+    }
+  }
+  for (int k = base + sub; k < np; k += TPI) {
+    const uint32_t o = lrow[k];
+    const double xj = *reinterpret_cast<const double*>(lb + o);
+    const double yj = *reinterpret_cast<const double*>(lb + o + 8);
+    const double zj = *reinterpret_cast<const double*>(lb + o + 16);
+    pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+  }
+}
+
+// LDS-tiled full-list kernel: one workgroup per block of consecutive
+// particles. The workgroup first stages into LDS (a) the positions of every
+// particle the block's rows reference (stage[soff .. soff+ns)), as x, y, z
+// triples, in a periodic box shifted to the image nearest the block's middle
+// particle, and (b) the block's ne per-entry LDS offsets (one contiguous,
+// coalesced range of `local`). Rows then run entirely out of LDS: the random
+// gathers of the CSR kernel become LDS reads and the list costs 2 bytes per
+// entry. In the unwrapped frame a pair difference equals its minimum image
+// whenever every particle of the block lies within box/2 - rc of the
+// reference point (rc = cutoff); blocks where that fails apply the minimum
+// image per entry instead. OFFS_LDS = false skips (b) and streams the
+// offsets from global memory instead (less LDS per workgroup, so more
+// resident workgroups per CU).
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
+__global__ void __launch_bounds__(1024)
+k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
+       const int4* __restrict__ desc, const int32_t* __restrict__ stage,
+       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach,
+       int nblocks_padded) {
+  extern __shared__ double lds[];
+  const int b = xcd_logical_block(nblocks_padded);
+  if (b >= nblocks) return;  // uniform over the workgroup
+  const int4 d0 = desc[2 * b];
+  const int4 d1 = desc[2 * b + 1];
+  const int i0 = d0.x, ni = d0.y, soff = d0.z, ns = d0.w;
+  const int eoff = d1.x, ne = d1.y;
+  char* lb = reinterpret_cast<char*>(lds);
+  const int off_area = (ns * 24 + 15) & ~15;
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if constexpr (PBC) pos.ld(i0 + ni / 2, cx, cy, cz);
+  // Staging issues all of a thread's loads of a pass before its LDS writes,
+  // so each thread has kStageR independent gathers in flight.
+  constexpr int kStageR = 4;
+  const int nt = static_cast<int>(blockDim.x);
+  for (int t0 = 0; t0 < ns; t0 += kStageR * nt) {
+    int jj[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; r++) {
+      const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
+      jj[r] = t < ns ? stage[soff + t] : i0;
+    }
+    double x[kStageR], y[kStageR], z[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; r++) pos.ld(jj[r], x[r], y[r], z[r]);
+#pragma unroll
+    for (int r = 0; r < kStageR; r++) {
+      const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
+      if (t < ns) {
+        unwrap_near<PBC>(x[r], y[r], z[r], cx, cy, cz, pp);
+        lds[3 * t] = x[r];
+        lds[3 * t + 1] = y[r];
+        lds[3 * t + 2] = z[r];
+      }
+    }
+  }
+  // The block's entry offsets, copied as 16-byte chunks from the 8-entry
+  // boundary at or below eoff (the device array is padded by 8 entries).
+  const int e0 = eoff & ~7;
+  if constexpr (OFFS_LDS) {
+    const int nchunks = (eoff + ne - e0 + 7) >> 3;
+    const uint4* src = reinterpret_cast<const uint4*>(local + e0);
+    uint4* dst = reinterpret_cast<uint4*>(lb + off_area);
+    for (int c0 = 0; c0 < nchunks; c0 += kStageR * nt) {
+      uint4 v[kStageR];
+#pragma unroll
+      for (int r = 0; r < kStageR; r++) {
+        const int c = c0 + r * nt + static_cast<int>(threadIdx.x);
+        if (c < nchunks) v[r] = src[c];
+      }
+#pragma unroll
+      for (int r = 0; r < kStageR; r++) {
+        const int c = c0 + r * nt + static_cast<int>(threadIdx.x);
+        if (c < nchunks) dst[c] = v[r];
+      }
+    }
+  }
+  const uint16_t* loff = reinterpret_cast<const uint16_t*>(lb + off_area) + (eoff - e0);
+  const int seg = static_cast<int>(threadIdx.x) / TPI;
+  const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
+  const bool active = seg < ni;
+  const int i = i0 + (active ? seg : 0);
+  double xi, yi, zi;
+  pos.ld(i, xi, yi, zi);
+  int far = 0;
+  if constexpr (PBC) {
+    unwrap_near<PBC>(xi, yi, zi, cx, cy, cz, pp);
+    far = active && (fabs(xi - cx) >= reach || fabs(yi - cy) >= reach || fabs(zi - cz) >= reach);
+  }
+  const int np = active ? nop[i] : 0;
+  const int rs = active ? ptr[i] - eoff : 0;
+  // Barrier after staging; also ORs the per-particle frame checks.
+  const int any_far = __syncthreads_or(far);
+  Accum<PREC> acc;
+  if (active) {
+    if constexpr (OFFS_LDS) {
+      if (PBC && any_far)
+        tile_row<TPI, U, PREC, true>(mom, lb, loff + rs, np, sub, xi, yi, zi, pp, acc);
+      else
+        tile_row<TPI, U, PREC, false>(mom, lb, loff + rs, np, sub, xi, yi, zi, pp, acc);
+    } else {
+      const uint16_t* grow = local + eoff + rs;
+      if (PBC && any_far)
+        tile_row_stream<TPI, U, PREC, true>(mom, lb, grow, np, sub, xi, yi, zi, pp, acc);
+      else
+        tile_row_stream<TPI, U, PREC, false>(mom, lb, grow, np, sub, xi, yi, zi, pp, acc);
+    }
+  }
+  const double fx = segment_sum<TPI>(acc.vx());
+  const double fy = segment_sum<TPI>(acc.vy());
+  const double fz = segment_sum<TPI>(acc.vz());
+  if (active && sub == 0) mom.add(i, fx, fy, fz);
+}
+
+inline int grid_blocks(int64_t threads, int block, int* padded) {
+  int64_t blocks = (threads + block - 1) / block;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+  *padded = static_cast<int>(blocks);
+  return static_cast<int>(blocks);
+}
+
+inline int launch_status(const char* what) {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LJGPU_OK
+                         : set_error(LJGPU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool HALF, bool CLAMP>
+int run_csr(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp, int block) {
+  int padded = 0;
+  const int blocks = grid_blocks(static_cast<int64_t>(lv.n) * TPI, block, &padded);
+  hipLaunchKernelGGL((k_csr<TPI, U, PREC, P, M, PBC, HALF, CLAMP>), dim3(blocks), dim3(block), 0,
+                     s, pos, mom, lv.n, lv.nop, lv.ptr, lv.list, lv.n_entries, pp, padded);
+  return launch_status("force kernel launch");
+}
+
+template <int U, int PREC, class P, class M, bool PBC, bool HALF>
+int run_ell(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp, int block) {
+  int padded = 0;
+  const int blocks = grid_blocks(lv.n, block, &padded);
+  hipLaunchKernelGGL((k_ell<U, PREC, P, M, PBC, HALF>), dim3(blocks), dim3(block), 0, s, pos, mom,
+                     lv.n, lv.nop, lv.list, lv.ell_width, pp, padded);
+  return launch_status("ELL force kernel launch");
+}
+
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
+int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp) {
+  const int threads = (lv.tile_particles * TPI + 63) / 64 * 64;
+  if (threads > 1024)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "tile_particles * lanes_per_particle must not exceed 1024");
+  const int padded = (lv.tile_blocks + kXcds - 1) / kXcds * kXcds;
+  const size_t lds_bytes = OFFS_LDS ? static_cast<size_t>(lv.tile_lds_bytes)
+                                    : static_cast<size_t>(lv.tile_max_stage) * 24;
+  const double reach = 0.5 * pp.box - sqrt(pp.cl2);
+  hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(padded), dim3(threads),
+                     lds_bytes, s, pos, mom, lv.nop, lv.ptr,
+                     reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
+                     lv.tile_blocks, pp, reach, padded);
+  return launch_status("tiled force kernel launch");
+}
+
+template <int TPI, int PREC, class P, class M, bool PBC>
+int run_tile_unroll(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+                    const LaunchConfig& lc) {
+  const bool o = lc.tile_offsets_lds;
+  switch (lc.unroll) {
+    case 1: return o ? run_tile<TPI, 1, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
+                     : run_tile<TPI, 1, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
+    case 2: return o ? run_tile<TPI, 2, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
+                     : run_tile<TPI, 2, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
+    case 4: return o ? run_tile<TPI, 4, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
+                     : run_tile<TPI, 4, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
+    default: return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
+  }
+}
+
+template <int PREC, class P, class M, bool PBC>
+int run_tile_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+                   const LaunchConfig& lc) {
+  switch (lc.lanes) {
+    case 4: return run_tile_unroll<4, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 8: return run_tile_unroll<8, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 16: return run_tile_unroll<16, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 32: return run_tile_unroll<32, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 64: return run_tile_unroll<64, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    default:
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                       "lanes_per_particle must be 4, 8, 16, 32 or 64 for the tiled kernel");
+  }
+}
+
+template <int TPI, int PREC, class P, class M, bool PBC, bool HALF, bool CLAMP>
+int run_unroll(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+               const LaunchConfig& lc) {
+  switch (lc.unroll) {
+    case 1: return run_csr<TPI, 1, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc.block);
+    case 2: return run_csr<TPI, 2, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc.block);
+    case 4: return run_csr<TPI, 4, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc.block);
+    default: return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
+  }
+}
+
+template <int PREC, class P, class M, bool PBC, bool HALF, bool CLAMP>
+int run_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+              const LaunchConfig& lc) {
+  if (lv.ell_width > 0) {
+    if constexpr (CLAMP) {
+      return set_error(LJGPU_ERR_UNSUPPORTED, "ELL list with clamping");
+    } else {
+      switch (lc.unroll) {
+        case 1: return run_ell<1, PREC, P, M, PBC, HALF>(s, pos, mom, lv, pp, lc.block);
+        case 2: return run_ell<2, PREC, P, M, PBC, HALF>(s, pos, mom, lv, pp, lc.block);
+        case 4: return run_ell<4, PREC, P, M, PBC, HALF>(s, pos, mom, lv, pp, lc.block);
+        default: return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
+      }
+    }
+  }
+  switch (lc.lanes) {
+    case 4: return run_unroll<4, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
+    case 8: return run_unroll<8, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
+    case 16: return run_unroll<16, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
+    case 32: return run_unroll<32, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
+    case 64: return run_unroll<64, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
+    default:
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                       "lanes_per_particle must be 4, 8, 16, 32 or 64 for a CSR list");
+  }
+}
+
+template <int PREC, class P, class M>
+int run_layout(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+               const LaunchConfig& lc) {
+  const bool pbc = pp.box > 0.0;
+  if (lc.clamp) {
+    // The reference-kernel-ABI entry: AoS, open box, FP64 only.
+    if constexpr (PREC == kF64 && std::is_same<P, PosAos>::value) {
+      if (pbc) return set_error(LJGPU_ERR_UNSUPPORTED, "clamped launch with periodic box");
+      return lc.half_atomic ? run_lanes<PREC, P, M, false, true, true>(s, pos, mom, lv, pp, lc)
+                            : run_lanes<PREC, P, M, false, false, true>(s, pos, mom, lv, pp, lc);
+    } else {
+      return set_error(LJGPU_ERR_UNSUPPORTED, "clamped launch outside AoS FP64");
+    }
+  }
+  if (lv.tile_blocks > 0 && !lc.half_atomic) {
+    return pbc ? run_tile_lanes<PREC, P, M, true>(s, pos, mom, lv, pp, lc)
+               : run_tile_lanes<PREC, P, M, false>(s, pos, mom, lv, pp, lc);
+  }
+  if (pbc) {
+    return lc.half_atomic ? run_lanes<PREC, P, M, true, true, false>(s, pos, mom, lv, pp, lc)
+                          : run_lanes<PREC, P, M, true, false, false>(s, pos, mom, lv, pp, lc);
+  }
+  return lc.half_atomic ? run_lanes<PREC, P, M, false, true, false>(s, pos, mom, lv, pp, lc)
+                        : run_lanes<PREC, P, M, false, false, false>(s, pos, mom, lv, pp, lc);
+}
+
+template <int PREC>
+int run_force(void* stream, const ListView& lv, const ForceArgs& fa, const LaunchConfig& lc) {
+  if (lv.n <= 0) return LJGPU_OK;
+  if (lc.block < 64 || lc.block > 1024 || (lc.block % 64) != 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "block size must be a multiple of 64 in [64, 1024]");
+  PairParams pp;
+  pp.c24 = 24.0 * fa.dt;
+  pp.c48 = 48.0 * fa.dt;
+  pp.cl2 = fa.cl2;
+  pp.box = fa.box > 0.0 ? fa.box : 0.0;
+  pp.inv_box = fa.box > 0.0 ? 1.0 / fa.box : 0.0;
+  pp.c24f = static_cast<float>(pp.c24);
+  pp.c48f = static_cast<float>(pp.c48);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fa.soa) {
+    PosSoa pos{fa.q[0], fa.q[1], fa.q[2]};
+    MomSoa mom{fa.p[0], fa.p[1], fa.p[2]};
+    return run_layout<PREC>(s, pos, mom, lv, pp, lc);
+  }
+  PosAos pos{fa.q[0]};
+  MomAos mom{fa.p[0]};
+  return run_layout<PREC>(s, pos, mom, lv, pp, lc);
+}
+
+}  // namespace detail
+}  // namespace ljgpu

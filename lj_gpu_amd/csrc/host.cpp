@@ -95,6 +95,82 @@ void expand_half_to_full(const HostCSR& half, HostCSR* full) {
   }
 }
 
+bool csr_is_row_ordered(const HostCSR& c) {
+  int64_t pos = 0;
+  for (int32_t i = 0; i < c.n; i++) {
+    if (c.ptr[i] != pos) return false;
+    pos += c.nop[i];
+  }
+  return true;
+}
+
+void csr_make_row_ordered(HostCSR* c) {
+  if (csr_is_row_ordered(*c)) return;
+  std::vector<int32_t> ptr(c->n), list;
+  list.reserve(c->list.size());
+  for (int32_t i = 0; i < c->n; i++) {
+    ptr[i] = static_cast<int32_t>(list.size());
+    list.insert(list.end(), c->list.begin() + c->ptr[i], c->list.begin() + c->ptr[i] + c->nop[i]);
+  }
+  c->ptr.swap(ptr);
+  c->list.swap(list);
+}
+
+bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out) {
+  const int32_t n = full.n;
+  out->tile_particles = tile_particles;
+  out->max_stage = 0;
+  out->lds_bytes = 0;
+  out->desc.clear();
+  out->stage.clear();
+  out->local.assign(full.list.size(), 0);
+  if (tile_particles <= 0 || !csr_is_row_ordered(full)) return false;
+  auto entries = [&](int32_t i0, int32_t ni) -> int64_t {
+    return static_cast<int64_t>(full.ptr[i0 + ni - 1]) + full.nop[i0 + ni - 1] - full.ptr[i0];
+  };
+  std::vector<int32_t> set;
+  auto collect = [&](int32_t i0, int32_t ni) {
+    set.clear();
+    for (int32_t i = i0; i < i0 + ni; i++) {
+      set.push_back(i);
+      const int32_t* r = full.list.data() + full.ptr[i];
+      set.insert(set.end(), r, r + full.nop[i]);
+    }
+    std::sort(set.begin(), set.end());
+    set.erase(std::unique(set.begin(), set.end()), set.end());
+  };
+  int32_t i0 = 0;
+  while (i0 < n) {
+    int32_t ni = std::min(tile_particles, n - i0);
+    collect(i0, ni);
+    while (tile_lds_bytes(static_cast<int64_t>(set.size()), entries(i0, ni)) > kTileLdsBudget &&
+           ni > 1) {
+      ni = std::max(1, ni / 2);
+      collect(i0, ni);
+    }
+    const int64_t ne = entries(i0, ni);
+    if (tile_lds_bytes(static_cast<int64_t>(set.size()), ne) > kTileLdsBudget) return false;
+    const int32_t soff = static_cast<int32_t>(out->stage.size());
+    const int32_t ns = static_cast<int32_t>(set.size());
+    out->desc.insert(out->desc.end(),
+                     {i0, ni, soff, ns, full.ptr[i0], static_cast<int32_t>(ne), 0, 0});
+    out->stage.insert(out->stage.end(), set.begin(), set.end());
+    out->max_stage = std::max(out->max_stage, ns);
+    out->lds_bytes = std::max(out->lds_bytes, static_cast<int32_t>(tile_lds_bytes(ns, ne)));
+    for (int32_t i = i0; i < i0 + ni; i++) {
+      const int32_t kp = full.ptr[i];
+      for (int32_t k = 0; k < full.nop[i]; k++) {
+        const int32_t j = full.list[kp + k];
+        const int32_t slot = static_cast<int32_t>(
+            std::lower_bound(set.begin(), set.end(), j) - set.begin());
+        out->local[kp + k] = static_cast<uint16_t>(slot * 24);
+      }
+    }
+    i0 += ni;
+  }
+  return true;
+}
+
 }  // namespace ljgpu
 
 using namespace ljgpu;
@@ -327,6 +403,37 @@ int ljgpu_random_shfl(int32_t n, const int32_t* nop, const int32_t* ptr, int32_t
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "random_shfl: bad argument");
   std::mt19937 mt(seed);
   for (int32_t i = 0; i < n; i++) std::shuffle(list + ptr[i], list + ptr[i] + nop[i], mt);
+  return LJGPU_OK;
+}
+
+int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,
+                     int64_t n_entries, int32_t tile_particles, int32_t* desc,
+                     int64_t desc_capacity, int32_t* stage, int64_t stage_capacity,
+                     uint16_t* local, int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
+  if (!n_blocks || !n_stage || !max_stage || tile_particles <= 0 || tile_particles > 1024)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_tiles: bad argument");
+  int rc = validate_csr(n, nop, ptr, list, n_entries, LJGPU_LIST_FULL);
+  if (rc) return rc;
+  HostCSR c;
+  c.n = n;
+  c.nop.assign(nop, nop + n);
+  c.ptr.assign(ptr, ptr + n);
+  c.list.assign(list, list + n_entries);
+  if (!csr_is_row_ordered(c))
+    return set_error(LJGPU_ERR_BAD_LIST, "make_tiles: rows must be stored in order");
+  HostTiles t;
+  if (!build_tiles(c, tile_particles, &t))
+    return set_error(LJGPU_ERR_CAPACITY, "make_tiles: a row does not fit one tile");
+  *n_blocks = static_cast<int32_t>(t.desc.size() / 8);
+  *n_stage = static_cast<int64_t>(t.stage.size());
+  *max_stage = t.max_stage;
+  if (!desc && !stage && !local) return LJGPU_OK;
+  if (!desc || !stage || !local || desc_capacity < static_cast<int64_t>(t.desc.size()) ||
+      stage_capacity < static_cast<int64_t>(t.stage.size()))
+    return set_error(LJGPU_ERR_CAPACITY, "make_tiles: output too small");
+  std::memcpy(desc, t.desc.data(), t.desc.size() * sizeof(int32_t));
+  std::memcpy(stage, t.stage.data(), t.stage.size() * sizeof(int32_t));
+  std::memcpy(local, t.local.data(), t.local.size() * sizeof(uint16_t));
   return LJGPU_OK;
 }
 

@@ -24,14 +24,18 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks_padded) {
   return (b % kXcds) * per + b / kXcds;
 }
 
+// Gathers use 32-bit unsigned byte offsets from a uniform base pointer, so
+// the compiler can emit the SGPR-base form of global loads (one VALU shift
+// per gather instead of 64-bit address arithmetic). Valid for N < 2^27.
 struct PosAos {
   const double* __restrict__ q;
   __device__ __forceinline__ void ld(int j, double& x, double& y, double& z) const {
-    const double* r = q + 4 * static_cast<int64_t>(j);
-    const double2 xy = *reinterpret_cast<const double2*>(r);
+    const uint32_t off = static_cast<uint32_t>(j) << 5;
+    const char* base = reinterpret_cast<const char*>(q);
+    const double2 xy = *reinterpret_cast<const double2*>(base + off);
     x = xy.x;
     y = xy.y;
-    z = r[2];
+    z = *reinterpret_cast<const double*>(base + off + 16u);
   }
 };
 
@@ -40,9 +44,10 @@ struct PosSoa {
   const double* __restrict__ y;
   const double* __restrict__ z;
   __device__ __forceinline__ void ld(int j, double& a, double& b, double& c) const {
-    a = x[j];
-    b = y[j];
-    c = z[j];
+    const uint32_t off = static_cast<uint32_t>(j) << 3;
+    a = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(x) + off);
+    b = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(y) + off);
+    c = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(z) + off);
   }
 };
 
@@ -101,24 +106,28 @@ __device__ __forceinline__ void min_image(double& dx, double& dy, double& dz,
   }
 }
 
-// 1/x in FP64: the hardware reciprocal estimate refined by two
-// Newton-Raphson steps r <- r + r*(1 - x*r) (quadratic convergence; the
-// refined value is within a few ulp of the correctly rounded quotient).
-// Valid for the finite, normal x of a pair distance.
+// 1/x in FP64: the hardware reciprocal estimate refined by one
+// Newton-Raphson step r <- r + r*(1 - x*r) (quadratic convergence). Measured
+// on gfx950 over x in [0.4, 10]: the estimate alone is good to about 2^-24
+// relative, one step to 11 ulp (2.4e-15), two steps to 0.5 ulp. One step
+// keeps pair forces within ~1e-14 of the divide, far inside the 1e-12
+// parity bar. Valid for the finite, normal x of a pair distance.
 __device__ __forceinline__ double rcp_f64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
-  r = fma(fma(-x, r, 1.0), r, r);
-  return r;
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(fma(-x, r, 1.0), r, r);
 }
 
 // FP64 force scalar times dt: ((24 r^6 - 48) / r^14) * dt rewritten with
 // s = 1/r2 as s^4 * (24 - 48 s^3) * dt (one reciprocal, no divide).
+// Beyond the cutoff only the high word of s is cleared: s becomes a
+// denormal of at most 2^-1042 whose third and fourth powers round to exactly
+// 0, so df is exactly 0 at the cost of one 32-bit select.
 __device__ __forceinline__ double lj_df_f64(double r2, const PairParams& pp) {
-  const double s = rcp_f64(r2);
+  double s = rcp_f64(r2);
+  const int hi = (r2 > pp.cl2) ? 0 : __double2hiint(s);
+  s = __hiloint2double(hi, __double2loint(s));
   const double s3 = s * s * s;
-  const double df = (s3 * s) * fma(-pp.c48, s3, pp.c24);
-  return (r2 > pp.cl2) ? 0.0 : df;
+  return (s3 * s) * fma(-pp.c48, s3, pp.c24);
 }
 
 // Mixed-precision force scalar: r2 (computed in FP64) rounded once to FP32,

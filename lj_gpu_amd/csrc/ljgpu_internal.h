@@ -39,7 +39,52 @@ struct ListView {
   const int32_t* list = nullptr;  // device CSR list, or ELL list (stride n)
   int64_t n_entries = 0;          // CSR length (for clamping)
   int32_t ell_width = 0;          // >0: `list` is ELL
+
+  // LDS-tiled form of a row-ordered CSR list (tile_blocks > 0 selects the
+  // tile kernel): block b covers particles [i0, i0+ni), whose entries are the
+  // contiguous list range [eoff, eoff+ne), and stages in LDS the positions
+  // of the ns particles stage[soff .. soff+ns) and the ne entry offsets.
+  // `local` is parallel to the CSR list and holds, per entry, the LDS byte
+  // offset (slot * 24) of its j in the block's position area.
+  int32_t tile_blocks = 0;
+  int32_t tile_particles = 0;       // max particles per block
+  int32_t tile_lds_bytes = 0;       // max LDS bytes per block (positions + offsets)
+  int32_t tile_max_stage = 0;       // max staged positions per block
+  const int32_t* tile_desc = nullptr;   // 8 ints per block: i0, ni, soff, ns, eoff, ne, 0, 0
+  const int32_t* tile_stage = nullptr;  // staged particle indices
+  const uint16_t* tile_local = nullptr; // per-entry LDS byte offsets
 };
+
+// Host-built LDS tiling of a full CSR list (see ListView).
+struct HostTiles {
+  int32_t tile_particles = 0;
+  int32_t max_stage = 0;
+  int32_t lds_bytes = 0;       // max over blocks of tile_lds_bytes(ns, ne)
+  std::vector<int32_t> desc;   // 8 per block
+  std::vector<int32_t> stage;
+  std::vector<uint16_t> local;
+};
+
+// LDS budget of one tile: staged positions (24 bytes each, addressed by
+// 16-bit byte offsets), then the tile's 16-bit entry offsets, copied in
+// 16-byte chunks from the 8-entry boundary below the first entry (up to 14
+// extra entries). The device copy of the offsets is padded by
+// kTileLocalPad entries so the last chunk stays inside the allocation.
+constexpr int32_t kTileLdsBudget = 65536;
+constexpr int32_t kTileLocalPad = 8;
+inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
+  return ((ns * 24 + 15) / 16) * 16 + ((ne + 14 + 7) / 8) * 16;
+}
+
+// True if rows are stored in order: pointer[i+1] == pointer[i] + nop[i].
+bool csr_is_row_ordered(const HostCSR& c);
+// Rewrite `c` so that its rows are stored in order (pointer = exclusive scan).
+void csr_make_row_ordered(HostCSR* c);
+
+// Split the particles of a row-ordered list into blocks of at most
+// `tile_particles` consecutive rows that fit the LDS budget (blocks shrink
+// where needed). Returns false if a single row does not fit.
+bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out);
 
 struct ForceArgs {
   bool soa = false;
@@ -49,8 +94,10 @@ struct ForceArgs {
 };
 
 struct LaunchConfig {
-  int lanes = 16;       // lanes per i-particle; 1 selects the ELL kernel
+  int lanes = 16;       // lanes per i-particle (CSR kernel); ELL lists use one
+  int unroll = 2;       // list entries per lane per pipelined batch: 1, 2 or 4
   int block = 256;      // threads per workgroup
+  bool tile_offsets_lds = false;  // tile kernel: stage entry offsets in LDS too
   bool half_atomic = false;  // list is a half list: scatter to j with atomics
   bool clamp = false;   // clamp every index read into range (unvalidated lists)
 };
@@ -58,7 +105,9 @@ struct LaunchConfig {
 // Enqueue one force evaluation on `stream`. Returns LJGPU_OK or an error.
 int launch_force_f64(void* stream, const ListView& lv, const ForceArgs& fa,
                      const LaunchConfig& lc);
-int launch_force_mixed(void* stream, const ListView& lv, const ForceArgs& fa,
-                       const LaunchConfig& lc, int accum);
+int launch_force_mixed_f64acc(void* stream, const ListView& lv, const ForceArgs& fa,
+                              const LaunchConfig& lc);
+int launch_force_mixed_kahan(void* stream, const ListView& lv, const ForceArgs& fa,
+                             const LaunchConfig& lc);
 
 }  // namespace ljgpu

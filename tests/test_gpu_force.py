@@ -30,10 +30,16 @@ def ctx():
 
 
 def run_aos(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, block=256,
-            half_mode=G.HALF_AUTO, mixed=None):
-    ctx.set_kernel_config(lanes, block)
+            half_mode=G.HALF_AUTO, mixed=None, unroll=2, tile=64):
+    ctx.set_kernel_config(lanes, block, unroll)
+    ctx.set_tiling(tile)
     ctx.set_half_mode(half_mode)
     ctx.set_list(csr)
+    info = ctx.list_info()
+    tiled = tile > 0 and lanes > 1 and not (csr.kind == G.LIST_HALF and half_mode == G.HALF_ATOMIC)
+    assert (info.tile_particles > 0) == tiled
+    if tiled:
+        assert info.tile_particles == min(tile, 1024 // lanes)
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)
     if mixed is None:
@@ -48,8 +54,9 @@ def run_aos(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, block=256,
 
 
 def run_soa(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, mixed=None,
-            half_mode=G.HALF_AUTO):
-    ctx.set_kernel_config(lanes, 256)
+            half_mode=G.HALF_AUTO, tile=64):
+    ctx.set_kernel_config(lanes, 256, 2)
+    ctx.set_tiling(tile)
     ctx.set_half_mode(half_mode)
     ctx.set_list(csr)
     q3 = to_soa(q)
@@ -68,17 +75,69 @@ def run_soa(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, mixed=None,
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
 @pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32, 64])
-def test_f64_full_list_aos(ctx, k, rho, lanes):
+@pytest.mark.parametrize("unroll", [1, 2, 4])
+@pytest.mark.parametrize("tile", [0, 64])
+def test_f64_full_list_aos(ctx, k, rho, lanes, unroll, tile):
+    if lanes == 1 and tile:
+        pytest.skip("the ELL kernel is never tiled")
     q, box, _, full = periodic(k, rho)
-    p = run_aos(ctx, q, full, box, lanes=lanes)
+    p = run_aos(ctx, q, full, box, lanes=lanes, unroll=unroll, tile=tile)
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
 
 @pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
+@pytest.mark.parametrize("lanes,unroll", [(8, 4), (16, 2), (4, 1), (32, 2)])
+def test_f64_tile_offsets_in_lds(ctx, k, rho, lanes, unroll):
+    q, box, _, full = periodic(k, rho)
+    ctx.set_tiling(64, True)
+    ctx.set_kernel_config(lanes, 256, unroll)
+    ctx.set_list(full)
+    dq = ctx.array(q)
+    dp = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, STEPS)
+    p = dp.download()
+    ctx.set_tiling(64, False)
+    assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
+
+
+@pytest.mark.parametrize("tile", [16, 32, 128, 1024])
+def test_f64_tile_sizes(ctx, tile):
+    q, box, _, full = periodic(12, 1.0)
+    p = run_aos(ctx, q, full, box, lanes=8, unroll=2, tile=tile)
+    assert normwise(p, oracle_momenta(12, 1.0)) < FP64_TOL
+
+
+def test_f64_tiles_in_small_periodic_box(ctx):
+    """Box 9.5 < 2 * (tile extent + cutoff): every tile takes the per-entry
+    minimum-image path."""
+    q, box, _, full = periodic(6, 1.0)
+    half = G.makepair(q, box, G.LIST_HALF)
+    p_ref = np.zeros_like(q)
+    import oracle as O
+    O.force_aos(O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half"), q, p_ref,
+                PBC_DT, O.CL2, box, steps=STEPS, method="sorted")
+    for lanes in (8, 32):
+        p = run_aos(ctx, q, full, box, lanes=lanes, unroll=2, tile=64)
+        assert normwise(p, p_ref) < FP64_TOL
+
+
+def test_f64_tiles_with_permuted_particle_order(ctx):
+    """Random particle order: tiles shrink to fit LDS; results unchanged."""
+    q, box, _, _ = periodic(12, 1.0)
+    perm = np.random.default_rng(5).permutation(q.shape[0])
+    qp = np.ascontiguousarray(q[perm])
+    full = G.makepair(qp, box, G.LIST_FULL)
+    p = run_aos(ctx, qp, full, box, lanes=8, unroll=2, tile=64)
+    assert ctx.list_info().tile_blocks > (full.n + 63) // 64
+    assert normwise(p, oracle_momenta(12, 1.0)[perm]) < FP64_TOL
+
+
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
 @pytest.mark.parametrize("mode", [G.HALF_ATOMIC, G.HALF_EXPAND])
-def test_f64_half_list_aos(ctx, k, rho, mode):
+@pytest.mark.parametrize("lanes,unroll", [(16, 2), (1, 4), (8, 1)])
+def test_f64_half_list_aos(ctx, k, rho, mode, lanes, unroll):
     q, box, half, _ = periodic(k, rho)
-    p = run_aos(ctx, q, half, box, half_mode=mode)
+    p = run_aos(ctx, q, half, box, half_mode=mode, lanes=lanes, unroll=unroll)
     assert ctx.list_info().half_mode == mode
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
@@ -192,10 +251,11 @@ def test_reference_kernel_abi_entry(ctx):
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
 @pytest.mark.parametrize("accum", [G.ACCUM_F64, G.ACCUM_KAHAN_F32])
-def test_mixed_force_error_bound(ctx, k, rho, accum):
+@pytest.mark.parametrize("lanes,unroll", [(16, 2), (1, 4), (64, 1), (8, 4)])
+def test_mixed_force_error_bound(ctx, k, rho, accum, lanes, unroll):
     q, box, _, full = periodic(k, rho)
     F = oracle_force(k, rho)
-    p = run_aos(ctx, q, full, box, dt=1.0, steps=1, mixed=accum)
+    p = run_aos(ctx, q, full, box, dt=1.0, steps=1, mixed=accum, lanes=lanes, unroll=unroll)
     err = normwise(p, F)
     rms = rms_normwise(p, F)
     print(f"mixed k={k} rho={rho} accum={accum}: max {err:.3e} rms {rms:.3e}")
@@ -233,9 +293,13 @@ def test_two_body_known_answers(ctx):
     p_in = _two_body(ctx, [3.0, 0.0, 0.0])  # r2 == CL2 exactly: kept
     assert p_in[0, 0] == pytest.approx(3.0 * (24.0 * 729.0 - 48.0) / 3.0 ** 14, rel=1e-14)
     assert np.abs(_two_body(ctx, [np.nextafter(3.0, 4.0), 0, 0])).max() == 0.0
+    # periodic pair: stored at x = 0.5 and 9.4 in a box of 10, minimum image -1.1
     p_wrap = _two_body(ctx, [10.0 - 1.1, 0, 0], box=10.0, origin=0.5)
-    p_dir = _two_body(ctx, [-1.1, 0, 0], origin=5.0)
-    np.testing.assert_allclose(p_wrap[:, :3], p_dir[:, :3], rtol=1e-14)
+    d = (0.5 + (10.0 - 1.1)) - 0.5 - 10.0
+    r2 = d * d
+    expected = (24.0 * r2 ** 3 - 48.0) / (r2 ** 7) * d
+    np.testing.assert_allclose(p_wrap[:, 0], [expected, -expected], rtol=1e-14)
+    assert np.abs(p_wrap[:, 1:3]).max() == 0.0
 
 
 def test_empty_rows_and_empty_list(ctx):
