@@ -52,6 +52,25 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lj_force_f64_N119164_rho1_traffic.json")
+
+
+def measured_traffic(info):
+    """HBM bytes per launch of the FP64 force kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (copy-calibrated) when they were
+    taken with the kernel configuration in use; None otherwise."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    cfg = t.get("kernel_config", {})
+    if (cfg.get("lanes_per_particle"), cfg.get("unroll"), cfg.get("tile_particles")) != \
+            (info.lanes_per_particle, info.unroll, info.tile_particles):
+        return None
+    return t.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(q, box, half, dt, budget_s: float) -> dict:
     """The oracle's force_sorted on one core over the same periodic list (a
     bounded sample of steps, about `budget_s` seconds)."""
@@ -79,9 +98,13 @@ def main() -> int:
     ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
     ap.add_argument("--unroll", type=int, default=0, help="entries per lane per batch (0: default)")
+    ap.add_argument("--tile", type=int, default=-1, help="particles per LDS tile (-1: default)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after timing, run three device-to-device copies of this many MiB "
+                         "(a known-size kernel that calibrates rocprofv3's byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -106,6 +129,8 @@ def main() -> int:
 
     ctx = G.Context(0)
     ctx.set_kernel_config(args.lanes, args.block, args.unroll)
+    if args.tile >= 0:
+        ctx.set_tiling(args.tile)
     ctx.set_list(full)
     info = ctx.list_info()
     dq = ctx.array(q)
@@ -170,6 +195,9 @@ def main() -> int:
             "particles": n, "density": args.density, "box": round(box, 6),
             "list_entries": e, "unique_pairs": pairs,
             "lanes_per_particle": info.lanes_per_particle,
+            "unroll": info.unroll,
+            "tile_particles": info.tile_particles,
+            "tile_max_stage": info.tile_max_stage,
         },
         "roofline": {
             "bound": "hbm",
@@ -177,7 +205,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(info),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": round(per_launch_s * 1e6, 3),
             "flop_rate_tflops": round(FLOP_PER_ENTRY * e / per_launch_s / 1e12, 3),
@@ -191,6 +219,15 @@ def main() -> int:
             "rms_rel_force_err_vs_f64": err_rms,
         },
     }
+    if args.copy_mib > 0:
+        nbytes = args.copy_mib << 20
+        a = ctx.zeros(nbytes // 8)
+        b = ctx.zeros(nbytes // 8)
+        for _ in range(3):
+            G.lib().ljgpu_memcpy_dtod(ctx.handle, b.ptr, a.ptr, nbytes)
+        ctx.sync()
+        a.free()
+        b.free()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(q, box, half, dt, args.cpu_seconds)
         log(f"cpu baseline: {result['cpu_baseline']['value']} Gpair/s")

@@ -160,10 +160,13 @@ int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_si
  * particles (one workgroup per tile); ELL lists and half lists evaluated
  * with atomics are never tiled. stage_offsets = 1 also stages each tile's
  * entry offsets in LDS (more LDS per workgroup); 0 streams them from HBM.
+ * resident_per_cu > 0 launches a persistent grid of that many workgroups per
+ * compute unit, each looping over tiles; 0 launches one workgroup per tile.
  * No reference counterpart: it plays the role of the reference's derived
  * transposed list (cuda/force_cuda.cu:229-240), a layout built once from the
  * CSR for the device. */
-int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets);
+int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
+                     int32_t resident_per_cu);
 
 /* ---- forces ------------------------------------------------------------ */
 /* p += F(q) * dt over the installed list, repeated `steps` times (the

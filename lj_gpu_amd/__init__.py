@@ -106,7 +106,7 @@ def lib() -> C.CDLL:
         "ljgpu_get_list_info": [_vp, C.POINTER(ListInfo)],
         "ljgpu_get_list_csr": [_vp, _vp, _vp, _vp, _i64],
         "ljgpu_set_half_mode": [_vp, C.c_int],
-        "ljgpu_set_tiling": [_vp, _i32, C.c_int],
+        "ljgpu_set_tiling": [_vp, _i32, C.c_int, _i32],
         "ljgpu_set_kernel_config": [_vp, C.c_int, C.c_int, C.c_int],
         "ljgpu_force_aos_f64": [_vp, _vp, _vp, _d, _d, _d, _i32],
         "ljgpu_force_soa_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32],
@@ -354,10 +354,14 @@ class Context:
     def set_half_mode(self, mode: int) -> None:
         _check(lib().ljgpu_set_half_mode(self.handle, mode))
 
-    def set_tiling(self, tile_particles: int, stage_offsets: bool = False) -> None:
+    def set_tiling(self, tile_particles: int, stage_offsets: bool = False,
+                   resident_per_cu: int = 0) -> None:
         """Particles per LDS tile (0: untiled global-gather kernels); with
-        stage_offsets the tile's entry offsets are staged in LDS as well."""
-        _check(lib().ljgpu_set_tiling(self.handle, tile_particles, int(stage_offsets)))
+        stage_offsets the tile's entry offsets are staged in LDS as well;
+        resident_per_cu > 0 runs a persistent grid of that many workgroups per
+        compute unit (0: one workgroup per tile)."""
+        _check(lib().ljgpu_set_tiling(self.handle, tile_particles, int(stage_offsets),
+                                      resident_per_cu))
 
     # lists
     def set_list(self, csr: CSR) -> None:

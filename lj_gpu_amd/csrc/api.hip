@@ -37,8 +37,10 @@ struct ljgpu_ctx {
   int32_t ell_width = 0;
 
   // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
-  int32_t tile_particles = 64;
+  int32_t tile_particles = 32;
   bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
+  int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
+  int cus = 0;                    // compute units of the device
   int32_t tiles_built_for = 0;  // tile_particles the device tiles were built with
   HostTiles tiles;
   int32_t* d_tile_desc = nullptr;
@@ -46,7 +48,7 @@ struct ljgpu_ctx {
   uint16_t* d_tile_local = nullptr;
 
   int lanes = 8;
-  int unroll = 2;
+  int unroll = 4;
   int block = 256;
 };
 
@@ -226,6 +228,8 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
   lc.unroll = ctx->unroll;
   lc.block = ctx->block;
   lc.tile_offsets_lds = ctx->tile_offsets_lds;
+  lc.tile_resident = ctx->tile_resident;
+  lc.cus = ctx->cus;
   lc.half_atomic = ctx->eval_half;
   lc.clamp = false;
   if (ctx->lanes == 1) {
@@ -288,6 +292,8 @@ int ljgpu_create(int device, ljgpu_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     ljgpu_destroy(ctx);
     return hip_fail(e, "context creation");
@@ -386,14 +392,18 @@ int ljgpu_timer_stop(ljgpu_ctx* ctx, float* ms) {
   return LJGPU_OK;
 }
 
-int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets) {
+int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
+                     int32_t resident_per_cu) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (tile_particles < 0 || tile_particles > 1024)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "tile_particles must be in [0, 1024]");
   if (stage_offsets != 0 && stage_offsets != 1)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "stage_offsets must be 0 or 1");
+  if (resident_per_cu < 0 || resident_per_cu > 64)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "resident_per_cu must be in [0, 64]");
   ctx->tile_particles = tile_particles;
   ctx->tile_offsets_lds = stage_offsets != 0;
+  ctx->tile_resident = resident_per_cu;
   return LJGPU_OK;
 }
 

@@ -254,36 +254,58 @@ __device__ __forceinline__ void tile_row(const M& mom, const char* lb, const uin
   }
 }
 
+// One full batch of a tile row: all U partner positions are read from LDS
+// before any is used, then the U entries are computed; the offsets buffer
+// `o` is refilled with the batch two ahead (`refill` >= 0) in between, so
+// the global offset loads have two batches of arithmetic to land.
+template <int TPI, int U, int PREC, bool MI, class M>
+__device__ __forceinline__ void tile_batch(const M& mom, const char* lb,
+                                           const uint16_t* __restrict__ lrow, uint32_t (&o)[U],
+                                           int refill, int sub, double xi, double yi, double zi,
+                                           const PairParams& pp, Accum<PREC>& acc) {
+  double xj[U], yj[U], zj[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    xj[u] = *reinterpret_cast<const double*>(lb + o[u]);
+    yj[u] = *reinterpret_cast<const double*>(lb + o[u] + 8);
+    zj[u] = *reinterpret_cast<const double*>(lb + o[u] + 16);
+  }
+  if (refill >= 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) o[u] = lrow[refill + u * TPI + sub];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    pair_compute<PREC, MI, false>(mom, 0, true, xj[u] - xi, yj[u] - yi, zj[u] - zi, pp, acc);
+}
+
 // As tile_row, with the row's entry offsets read from global memory
-// (`lrow` in HBM): the offsets of the next full batch are loaded while the
-// current batch is computed.
+// (`lrow` in HBM) through two alternating offset buffers, each refilled two
+// batches ahead.
 template <int TPI, int U, int PREC, bool MI, class M>
 __device__ __forceinline__ void tile_row_stream(const M& mom, const char* lb,
                                                 const uint16_t* __restrict__ lrow, int np,
                                                 int sub, double xi, double yi, double zi,
                                                 const PairParams& pp, Accum<PREC>& acc) {
   constexpr int kB = U * TPI;
-  uint32_t on[U];
+  uint32_t oa[U], ob[U];
   if (kB <= np) {
 #pragma unroll
-    for (int u = 0; u < U; u++) on[u] = lrow[u * TPI + sub];
+    for (int u = 0; u < U; u++) oa[u] = lrow[u * TPI + sub];
+  }
+  if (2 * kB <= np) {
+#pragma unroll
+    for (int u = 0; u < U; u++) ob[u] = lrow[kB + u * TPI + sub];
   }
   int base = 0;
-  for (; base + kB <= np; base += kB) {
-    uint32_t off[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) off[u] = on[u];
-    if (base + 2 * kB <= np) {
-#pragma unroll
-      for (int u = 0; u < U; u++) on[u] = lrow[base + kB + u * TPI + sub];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const double xj = *reinterpret_cast<const double*>(lb + off[u]);
-      const double yj = *reinterpret_cast<const double*>(lb + off[u] + 8);
-      const double zj = *reinterpret_cast<const double*>(lb + off[u] + 16);
-      pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
-    }
+  while (base + kB <= np) {
+    tile_batch<TPI, U, PREC, MI>(mom, lb, lrow, oa, base + 3 * kB <= np ? base + 2 * kB : -1,
+                                 sub, xi, yi, zi, pp, acc);
+    base += kB;
+    if (base + kB > np) break;
+    tile_batch<TPI, U, PREC, MI>(mom, lb, lrow, ob, base + 3 * kB <= np ? base + 2 * kB : -1,
+                                 sub, xi, yi, zi, pp, acc);
+    base += kB;
   }
   for (int k = base + sub; k < np; k += TPI) {
     const uint32_t o = lrow[k];
@@ -307,20 +329,26 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const char* lb,
 // image per entry instead. OFFS_LDS = false skips (b) and streams the
 // offsets from global memory instead (less LDS per workgroup, so more
 // resident workgroups per CU).
+//
+// Tiles are assigned XCD-aware: the workgroups on XCD x (blockIdx % 8 == x)
+// walk the contiguous tile range [x*per, (x+1)*per) with a stride of
+// gridDim/8. With one workgroup per tile each does a single tile; with a
+// smaller (persistent) grid each loops over several, so resident workgroups
+// drift out of phase and one's staging and memory stalls overlap another's
+// arithmetic.
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
-__global__ void __launch_bounds__(1024)
-k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
-       const int4* __restrict__ desc, const int32_t* __restrict__ stage,
-       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach,
-       int nblocks_padded) {
-  extern __shared__ double lds[];
-  const int b = xcd_logical_block(nblocks_padded);
-  if (b >= nblocks) return;  // uniform over the workgroup
+__device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restrict__ nop,
+                                          const int32_t* __restrict__ ptr,
+                                          const int4* __restrict__ desc,
+                                          const int32_t* __restrict__ stage,
+                                          const uint16_t* __restrict__ local, int b,
+                                          const PairParams& pp, double reach, char* lb,
+                                          int* flag) {
   const int4 d0 = desc[2 * b];
   const int4 d1 = desc[2 * b + 1];
   const int i0 = d0.x, ni = d0.y, soff = d0.z, ns = d0.w;
   const int eoff = d1.x, ne = d1.y;
-  char* lb = reinterpret_cast<char*>(lds);
+  double* lds = reinterpret_cast<double*>(lb);
   const int off_area = (ns * 24 + 15) & ~15;
   double cx = 0.0, cy = 0.0, cz = 0.0;
   if constexpr (PBC) pos.ld(i0 + ni / 2, cx, cy, cz);
@@ -384,8 +412,12 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   }
   const int np = active ? nop[i] : 0;
   const int rs = active ? ptr[i] - eoff : 0;
-  // Barrier after staging; also ORs the per-particle frame checks.
-  const int any_far = __syncthreads_or(far);
+  // The frame check ORed over the workgroup through one LDS word: threads
+  // that fail it write this tile's index, which no earlier tile of this
+  // workgroup used, so the word needs no reset.
+  if (far) *flag = b;
+  __syncthreads();  // staging complete
+  const bool any_far = PBC && *flag == b;
   Accum<PREC> acc;
   if (active) {
     if constexpr (OFFS_LDS) {
@@ -405,6 +437,30 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   const double fy = segment_sum<TPI>(acc.vy());
   const double fz = segment_sum<TPI>(acc.vz());
   if (active && sub == 0) mom.add(i, fx, fy, fz);
+}
+
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
+__global__ void __launch_bounds__(1024)
+k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
+       const int4* __restrict__ desc, const int32_t* __restrict__ stage,
+       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach) {
+  // Dynamic LDS only (no static LDS, so its base is address 0 and constant
+  // offsets fold into the LDS instructions): a 16-byte header holding the
+  // frame-check word, then the tile's positions (and offsets).
+  extern __shared__ double lds[];
+  int* flag = reinterpret_cast<int*>(lds);
+  char* lb = reinterpret_cast<char*>(lds) + kTileLdsHeader;
+  if (threadIdx.x == 0) *flag = -1;
+  __syncthreads();
+  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+  const int per = (nblocks + kXcds - 1) / kXcds;
+  const int stride = static_cast<int>(gridDim.x) / kXcds;
+  const int t_end = min(nblocks, (xcd + 1) * per);
+  for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
+    tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, ptr, desc, stage, local, b, pp,
+                                                 reach, lb, flag);
+    __syncthreads();  // the next tile overwrites this tile's LDS
+  }
 }
 
 inline int grid_blocks(int64_t threads, int block, int* padded) {
@@ -439,19 +495,25 @@ int run_ell(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& p
 }
 
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
-int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp) {
+int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+             const LaunchConfig& lc) {
   const int threads = (lv.tile_particles * TPI + 63) / 64 * 64;
   if (threads > 1024)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT,
                      "tile_particles * lanes_per_particle must not exceed 1024");
-  const int padded = (lv.tile_blocks + kXcds - 1) / kXcds * kXcds;
-  const size_t lds_bytes = OFFS_LDS ? static_cast<size_t>(lv.tile_lds_bytes)
-                                    : static_cast<size_t>(lv.tile_max_stage) * 24;
+  int grid = (lv.tile_blocks + kXcds - 1) / kXcds * kXcds;
+  if (lc.tile_resident > 0 && lc.cus > 0) {
+    const int persistent = (lc.tile_resident * lc.cus + kXcds - 1) / kXcds * kXcds;
+    grid = std::min(grid, persistent);
+  }
+  const size_t lds_bytes =
+      kTileLdsHeader + (OFFS_LDS ? static_cast<size_t>(lv.tile_lds_bytes)
+                                 : static_cast<size_t>(lv.tile_max_stage) * 24);
   const double reach = 0.5 * pp.box - sqrt(pp.cl2);
-  hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(padded), dim3(threads),
+  hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(grid), dim3(threads),
                      lds_bytes, s, pos, mom, lv.nop, lv.ptr,
                      reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
-                     lv.tile_blocks, pp, reach, padded);
+                     lv.tile_blocks, pp, reach);
   return launch_status("tiled force kernel launch");
 }
 
@@ -460,12 +522,12 @@ int run_tile_unroll(hipStream_t s, P pos, M mom, const ListView& lv, const PairP
                     const LaunchConfig& lc) {
   const bool o = lc.tile_offsets_lds;
   switch (lc.unroll) {
-    case 1: return o ? run_tile<TPI, 1, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
-                     : run_tile<TPI, 1, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
-    case 2: return o ? run_tile<TPI, 2, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
-                     : run_tile<TPI, 2, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
-    case 4: return o ? run_tile<TPI, 4, PREC, P, M, PBC, true>(s, pos, mom, lv, pp)
-                     : run_tile<TPI, 4, PREC, P, M, PBC, false>(s, pos, mom, lv, pp);
+    case 1: return o ? run_tile<TPI, 1, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
+                     : run_tile<TPI, 1, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
+    case 2: return o ? run_tile<TPI, 2, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
+                     : run_tile<TPI, 2, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
+    case 4: return o ? run_tile<TPI, 4, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
+                     : run_tile<TPI, 4, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
     default: return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
   }
 }

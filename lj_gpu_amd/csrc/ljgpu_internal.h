@@ -70,7 +70,8 @@ struct HostTiles {
 // 16-byte chunks from the 8-entry boundary below the first entry (up to 14
 // extra entries). The device copy of the offsets is padded by
 // kTileLocalPad entries so the last chunk stays inside the allocation.
-constexpr int32_t kTileLdsBudget = 65536;
+constexpr int32_t kTileLdsHeader = 16;  // kernel's own word(s) ahead of the tile data
+constexpr int32_t kTileLdsBudget = 65536 - kTileLdsHeader;
 constexpr int32_t kTileLocalPad = 8;
 inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
   return ((ns * 24 + 15) / 16) * 16 + ((ne + 14 + 7) / 8) * 16;
@@ -98,6 +99,8 @@ struct LaunchConfig {
   int unroll = 2;       // list entries per lane per pipelined batch: 1, 2 or 4
   int block = 256;      // threads per workgroup
   bool tile_offsets_lds = false;  // tile kernel: stage entry offsets in LDS too
+  int tile_resident = 0;     // tile kernel: persistent grid of this many workgroups per CU (0: one per tile)
+  int cus = 0;               // compute units of the device
   bool half_atomic = false;  // list is a half list: scatter to j with atomics
   bool clamp = false;   // clamp every index read into range (unvalidated lists)
 };

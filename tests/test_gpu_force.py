@@ -100,6 +100,22 @@ def test_f64_tile_offsets_in_lds(ctx, k, rho, lanes, unroll):
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
 
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0)])
+@pytest.mark.parametrize("resident", [1, 3, 8])
+@pytest.mark.parametrize("stage_offsets", [False, True])
+def test_f64_tile_persistent_grid(ctx, k, rho, resident, stage_offsets):
+    q, box, _, full = periodic(k, rho)
+    ctx.set_tiling(32, stage_offsets, resident)
+    ctx.set_kernel_config(8, 256, 4)
+    ctx.set_list(full)
+    dq = ctx.array(q)
+    dp = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, STEPS)
+    p = dp.download()
+    ctx.set_tiling(32, False, 0)
+    assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
+
+
 @pytest.mark.parametrize("tile", [16, 32, 128, 1024])
 def test_f64_tile_sizes(ctx, tile):
     q, box, _, full = periodic(12, 1.0)
