@@ -262,6 +262,38 @@ int ljgpu_make_tiles(int32_t particle_number, const int32_t* number_of_partners,
                      int32_t* stage, int64_t stage_capacity, uint16_t* local,
                      int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage);
 
+/* ---- pair-list caches (host files, no device needed) ---------------------- */
+/* pair.dat, the binary cache of the CPU programs (savepair / loadpair,
+ * cpu_ref/force_aos.cpp:603-620; cpu_ref/force_soa.cpp:360-376): int32
+ * number_of_pairs, int32 number_of_partners[n_capacity], int32
+ * i_particles[pair_capacity], int32 j_particles[pair_capacity], unused tails
+ * zero. n_capacity / pair_capacity are the reference's compile-time array
+ * sizes; 0 selects its values (400000 and 30 * n_capacity). The writer takes
+ * a half list and writes its pairs in CSR storage order (for a list from
+ * makepair + sortpair: makepair's order), byte-identical to savepair(). The
+ * reader rebuilds the CSR as loadpair() + sortpair() do; pass a null
+ * sorted_list to query *n_entries first. */
+int ljgpu_pair_dat_write(const char* path, int32_t particle_number,
+                         const int32_t* number_of_partners, const int32_t* pointer,
+                         const int32_t* sorted_list, int64_t n_entries, int32_t n_capacity,
+                         int64_t pair_capacity);
+int ljgpu_pair_dat_read(const char* path, int32_t particle_number, int32_t n_capacity,
+                        int64_t pair_capacity, int32_t* number_of_partners, int32_t* pointer,
+                        int32_t* sorted_list, int64_t capacity, int64_t* n_entries);
+
+/* .cache_pair_half.dat / .cache_pair_all.dat, the GPU harness's text cache
+ * (makepaircache / loadpair, cuda/force_cuda.cu:165-227): "N E", then N lines
+ * "number_of_partners pointer", then E lines "sorted_list". The reader refuses
+ * a cache for another particle number, as the reference does, and validates
+ * the list (check_loadedpair, strict form) as `kind`. Pass a null sorted_list
+ * to query *n_entries (number_of_partners and pointer are filled). */
+int ljgpu_pair_cache_write(const char* path, int32_t particle_number,
+                           const int32_t* number_of_partners, const int32_t* pointer,
+                           const int32_t* sorted_list, int64_t n_entries);
+int ljgpu_pair_cache_read(const char* path, int32_t particle_number, int kind,
+                          int32_t* number_of_partners, int32_t* pointer, int32_t* sorted_list,
+                          int64_t capacity, int64_t* n_entries);
+
 #ifdef __cplusplus
 }
 #endif

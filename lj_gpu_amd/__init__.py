@@ -11,6 +11,8 @@ cuda/force_cuda.cu and cpu_ref/force_aos.cpp) over the C ABI of
     cuda_ptr<T>                  -> DeviceArray
     measure(kernel, ...)         -> Context.force_* (+ Context.timer_*)
     check_loadedpair()           -> validate_csr()
+    savepair() / loadpair()      -> write_pair_dat() / read_pair_dat()
+    makepaircache() / loadpair() -> write_pair_cache() / read_pair_cache()
 
 All compute runs in the HIP kernels of libljgpu.so; if the library is
 missing, importing the device API raises (there is no CPU fallback).
@@ -62,6 +64,7 @@ class LJGPUError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"ljgpu error {code}: {msg}")
         self.code = code
+        self.message = msg
 
 
 class ListInfo(C.Structure):
@@ -121,6 +124,12 @@ def lib() -> C.CDLL:
         "ljgpu_make_transposed": [_i32, _vp, _vp, _vp, _i32, _vp],
         "ljgpu_make_tiles": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64, _vp,
                              C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_pair_dat_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64, _i32, _i64],
+        "ljgpu_pair_dat_read": [C.c_char_p, _i32, _i32, _i64, _vp, _vp, _vp, _i64,
+                                C.POINTER(_i64)],
+        "ljgpu_pair_cache_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64],
+        "ljgpu_pair_cache_read": [C.c_char_p, _i32, C.c_int, _vp, _vp, _vp, _i64,
+                                  C.POINTER(_i64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -237,6 +246,51 @@ def make_transposed(csr: CSR, width: int | None = None) -> np.ndarray:
     _check(lib().ljgpu_make_transposed(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
                                        _ptr(csr.sorted_list), width, _ptr(out)))
     return out[: csr.n * width]
+
+
+def write_pair_dat(path: str, csr: CSR, n_capacity: int = 0, pair_capacity: int = 0) -> None:
+    """Write a half list as the CPU programs' binary pair.dat
+    (cpu_ref/force_aos.cpp:612-620); 0 capacities are the reference's."""
+    _check(lib().ljgpu_pair_dat_write(os.fsencode(path), csr.n, _ptr(csr.number_of_partners),
+                                      _ptr(csr.pointer), _ptr(csr.sorted_list), csr.n_entries,
+                                      n_capacity, pair_capacity))
+
+
+def read_pair_dat(path: str, particle_number: int, n_capacity: int = 0,
+                  pair_capacity: int = 0) -> CSR:
+    """Read pair.dat and rebuild the CSR as loadpair() + sortpair() do
+    (cpu_ref/force_aos.cpp:100-120,603-610)."""
+    n = particle_number
+    nop = np.zeros(n, np.int32)
+    ptr = np.zeros(n, np.int32)
+    e = _i64(0)
+    args = (os.fsencode(path), n, n_capacity, pair_capacity, _ptr(nop), _ptr(ptr))
+    _check(lib().ljgpu_pair_dat_read(*args, None, 0, C.byref(e)))
+    lst = np.zeros(max(e.value, 1), np.int32)
+    _check(lib().ljgpu_pair_dat_read(*args, _ptr(lst), lst.size, C.byref(e)))
+    return CSR(nop, ptr, lst[: e.value].copy(), LIST_HALF)
+
+
+def write_pair_cache(path: str, csr: CSR) -> None:
+    """Write the GPU harness's text cache (.cache_pair_{half,all}.dat,
+    cuda/force_cuda.cu:165-177)."""
+    _check(lib().ljgpu_pair_cache_write(os.fsencode(path), csr.n, _ptr(csr.number_of_partners),
+                                        _ptr(csr.pointer), _ptr(csr.sorted_list), csr.n_entries))
+
+
+def read_pair_cache(path: str, particle_number: int, kind: int) -> CSR:
+    """Read a text pair cache (cuda/force_cuda.cu:203-227); refuses a cache
+    for another particle number and validates the list as `kind`."""
+    n = particle_number
+    nop = np.zeros(n, np.int32)
+    ptr = np.zeros(n, np.int32)
+    e = _i64(0)
+    _check(lib().ljgpu_pair_cache_read(os.fsencode(path), n, kind, _ptr(nop), _ptr(ptr), None, 0,
+                                       C.byref(e)))
+    lst = np.zeros(max(e.value, 1), np.int32)
+    _check(lib().ljgpu_pair_cache_read(os.fsencode(path), n, kind, _ptr(nop), _ptr(ptr),
+                                       _ptr(lst), lst.size, C.byref(e)))
+    return CSR(nop, ptr, lst[: e.value].copy(), kind)
 
 
 @dataclass

@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["api.hip", "force_f64.hip", "force_mixed.hip", "force_kahan.hip"]
-HOST_SOURCES = ["host.cpp"]
+HOST_SOURCES = ["host.cpp", "paircache.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -34,6 +34,8 @@ extern double L;
 
 void init(void);
 void makepair(void);
+void savepair(void);
+void loadpair(void);
 void sortpair(void);
 void force_pair(void);
 void force_sorted(void);
@@ -120,6 +122,23 @@ void ljref_get_momenta(double* p4) {
 int ljref_makepair(void) {
   number_of_pairs = 0;
   makepair();
+  sortpair();
+  return number_of_pairs;
+}
+
+// The reference's savepair(): makepair() then the binary pair.dat in the
+// current directory (cpu_ref/force_aos.cpp:612-620), from an empty pair
+// count as at program start. Returns the number of pairs.
+int ljref_savepair(void) {
+  number_of_pairs = 0;
+  savepair();
+  return number_of_pairs;
+}
+
+// The reference's loadpair() of ./pair.dat then sortpair()
+// (cpu_ref/force_aos.cpp:603-610,644-645). Returns the number of pairs.
+int ljref_loadpair(void) {
+  loadpair();
   sortpair();
   return number_of_pairs;
 }

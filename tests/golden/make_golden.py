@@ -15,6 +15,9 @@ stores what they compute as small .npz fixtures (plain arrays, no pickles):
                            100 force_sorted steps, full CSR
   ref_soa_L50_rho0.5.npz   force_soa.cpp: force_pair / force_sorted /
                            force_intrin on its own init()
+  ref_pairdat.json         size and sha256 of the binary pair.dat that the
+                           reference's savepair() writes for init() with
+                           L=12 and L=50 (density 0.5)
   ref_aos_open_rho1.npz    density 1.0 (N=119,164): positions from the
                            restated lattice generator (the reference's
                            density is a compile-time constant), pair list and
@@ -113,13 +116,48 @@ def run_config(name: str) -> None:
     print(name, "n", n, "pairs", npairs)
 
 
+def run_pairdat(box: float) -> dict:
+    """The reference's savepair() on its own init() with side `box`, in a
+    scratch directory: size and sha256 of the pair.dat it writes."""
+    import tempfile
+
+    import oracle as O
+
+    R = O.ref_lib("aos")
+    if R is None:
+        raise SystemExit("oracle/_ref not built: run `make -C oracle ref`")
+    n = R.ljref_init(box)
+    with tempfile.TemporaryDirectory() as d:
+        cwd = os.getcwd()
+        os.chdir(d)
+        try:
+            npairs = R.ljref_savepair()
+            blob = open("pair.dat", "rb").read()
+        finally:
+            os.chdir(cwd)
+    return {"box": box, "density": 0.5, "n": n, "number_of_pairs": npairs,
+            "size": len(blob), "sha256": hashlib.sha256(blob).hexdigest()}
+
+
 def main() -> None:
     if len(sys.argv) > 1:
-        run_config(sys.argv[1])
+        if sys.argv[1].startswith("pairdat"):
+            print(json.dumps(run_pairdat(float(sys.argv[1].split(":")[1]))))
+        else:
+            run_config(sys.argv[1])
         return
     for name in ("ref_aos_L50_rho0.5", "ref_aos_L12_rho0.5", "ref_soa_L50_rho0.5",
                  "ref_aos_open_rho1"):
         subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True)
+    # pair.dat written by the reference's savepair() (cpu_ref/force_aos.cpp:612-620).
+    pairdat = {}
+    for box in (12.0, 50.0):
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), f"pairdat:{box}"],
+                             check=True, capture_output=True, text=True).stdout
+        pairdat[f"L{int(box)}_rho0.5"] = json.loads(out.strip().splitlines()[-1])
+    with open(os.path.join(OUT, "ref_pairdat.json"), "w") as f:
+        json.dump(pairdat, f, indent=1)
+    print("ref_pairdat", pairdat)
 
 
 if __name__ == "__main__":
