@@ -136,9 +136,25 @@ int ljgpu_set_list_transposed(ljgpu_ctx* ctx, int32_t particle_number,
 
 int ljgpu_get_list_info(ljgpu_ctx* ctx, ljgpu_list_info* info);
 
-/* Copy the installed list back as CSR (caller buffers on the host). */
+/* Copy the evaluated list back as CSR (caller buffers on the host): the list
+ * as installed, with rows stored in order, or for a half list in
+ * LJGPU_HALF_EXPAND mode its expanded full list. */
 int ljgpu_get_list_csr(ljgpu_ctx* ctx, int32_t* number_of_partners, int32_t* pointer,
                        int32_t* sorted_list, int64_t capacity);
+
+/* Build the Verlet list on the device from device positions and install it
+ * (no host round trip): the reference's makepair() + sortpair()
+ * (cpu_ref/force_aos.cpp:100-140; full list cuda/force_cuda.cu:122-163) as a
+ * cell list, with the identical pair set and CSR order (predicate
+ * r2 = (dx*dx + dy*dy) + dz*dz < search_length^2 evaluated unfused, rows in
+ * ascending i, partners ascending; minimum image for box > 0, positions in
+ * [0, box)). q is AoS double[N][4] (16-byte aligned) or three SoA arrays.
+ * A half list in LJGPU_HALF_EXPAND mode is built directly as its full list.
+ * Every derived layout (tiles, ELL) is built on the device as well. */
+int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t particle_number, double box,
+                     double search_length, int kind);
+int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
+                         int32_t particle_number, double box, double search_length, int kind);
 
 /* Half-list evaluation mode (LJGPU_HALF_*); takes effect on the next
  * ljgpu_set_list_* call. */
@@ -167,6 +183,16 @@ int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_si
  * CSR for the device. */
 int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
                      int32_t resident_per_cu);
+
+/* Copy the LDS tiling in use back to the host, in the layout
+ * ljgpu_make_tiles produces (derived on the device for the current kernel
+ * configuration; *n_blocks = 0 when the kernels run untiled). Pass null
+ * outputs to query the sizes first; local needs one value per entry of the
+ * evaluated list. Inspection only; the tiling's counterpart in the reference
+ * is the derived transposed list (cuda/force_cuda.cu:229-240). */
+int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_t* stage,
+                    int64_t stage_capacity, uint16_t* local, int64_t local_capacity,
+                    int32_t* n_blocks, int64_t* n_stage);
 
 /* ---- forces ------------------------------------------------------------ */
 /* p += F(q) * dt over the installed list, repeated `steps` times (the

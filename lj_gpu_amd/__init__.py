@@ -124,6 +124,10 @@ def lib() -> C.CDLL:
         "ljgpu_make_transposed": [_i32, _vp, _vp, _vp, _i32, _vp],
         "ljgpu_make_tiles": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64, _vp,
                              C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_build_list": [_vp, _vp, _i32, _d, _d, C.c_int],
+        "ljgpu_build_list_soa": [_vp, _vp, _vp, _vp, _i32, _d, _d, C.c_int],
+        "ljgpu_get_tiles": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i32),
+                            C.POINTER(_i64)],
         "ljgpu_pair_dat_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64, _i32, _i64],
         "ljgpu_pair_dat_read": [C.c_char_p, _i32, _i32, _i64, _vp, _vp, _vp, _i64,
                                 C.POINTER(_i64)],
@@ -432,6 +436,33 @@ class Context:
                             kind: int) -> None:
         _check(lib().ljgpu_set_list_transposed(self.handle, n, _ptr(nop), _ptr(transposed),
                                                width, kind, MEM_HOST))
+
+    def build_list(self, q: DeviceArray, n: int, box: float = 0.0, kind: int = LIST_HALF,
+                   search_length: float = SEARCH_LENGTH) -> None:
+        """Build the Verlet list on the device from device AoS positions
+        (the reference's makepair + sortpair, exact pair set and order)."""
+        _check(lib().ljgpu_build_list(self.handle, q.ptr, n, box, search_length, kind))
+
+    def build_list_soa(self, q3: list, n: int, box: float = 0.0, kind: int = LIST_HALF,
+                       search_length: float = SEARCH_LENGTH) -> None:
+        _check(lib().ljgpu_build_list_soa(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, n, box,
+                                          search_length, kind))
+
+    def get_tiles(self) -> Tiles | None:
+        """The LDS tiling in use, copied back (None when untiled)."""
+        nb, ns = _i32(0), _i64(0)
+        _check(lib().ljgpu_get_tiles(self.handle, None, 0, None, 0, None, 0, C.byref(nb),
+                                     C.byref(ns)))
+        if nb.value == 0:
+            return None
+        e = int(self.get_list().n_entries)
+        desc = np.zeros(8 * nb.value, np.int32)
+        stage = np.zeros(max(ns.value, 1), np.int32)
+        local = np.zeros(max(e, 1), np.uint16)
+        _check(lib().ljgpu_get_tiles(self.handle, _ptr(desc), desc.size, _ptr(stage), stage.size,
+                                     _ptr(local), local.size, C.byref(nb), C.byref(ns)))
+        d = desc.reshape(-1, 8)
+        return Tiles(d, stage[: ns.value], local[:e], int(d[:, 3].max()))
 
     def list_info(self) -> ListInfo:
         info = ListInfo()

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ljgpu_internal.h"
+#include "lists.h"
 
 using namespace ljgpu;
 
@@ -18,34 +19,32 @@ struct ljgpu_ctx {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   bool timer_running = false;
 
-  // List as given by the caller.
+  // List as given by the caller (or built by ljgpu_build_list).
   int32_t n = 0;
   int kind = -1;
   int64_t n_entries = 0;
   int32_t max_np = 0, min_np = 0;
 
-  // List as evaluated on the device: CSR (full, or half for atomic mode).
+  // List as evaluated on the device: row-ordered CSR (the full list, or the
+  // half list itself for atomic mode), and the layouts derived from it.
   int half_mode_req = LJGPU_HALF_AUTO;
   int half_mode = 0;
   bool eval_half = false;
-  HostCSR eval;  // host copy (used for derived layouts and get_list)
-  int32_t* d_nop = nullptr;
-  int32_t* d_ptr = nullptr;
-  int32_t* d_list = nullptr;
-  int64_t d_list_len = 0;
-  int32_t* d_ell = nullptr;
+  DevCSR eval;
+  DevCSR staging;  // a caller's host list on its way to `eval`
+  DevMem ell;
   int32_t ell_width = 0;
+  bool ell_ready = false;
 
   // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
   int32_t tile_particles = 32;
   bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
   int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
   int cus = 0;                    // compute units of the device
-  int32_t tiles_built_for = 0;  // tile_particles the device tiles were built with
-  HostTiles tiles;
-  int32_t* d_tile_desc = nullptr;
-  int32_t* d_tile_stage = nullptr;
-  uint16_t* d_tile_local = nullptr;
+  int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
+  DevTiles tiles;
+
+  ListWork work;
 
   int lanes = 8;
   int unroll = 4;
@@ -65,67 +64,43 @@ int hip_fail(hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
+#define TRY(call)         \
+  do {                    \
+    int rc_ = (call);     \
+    if (rc_) return rc_;  \
+  } while (0)
+
 int use_device(ljgpu_ctx* ctx) {
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   return LJGPU_OK;
 }
 
-void free_tiles(ljgpu_ctx* ctx) {
-  if (ctx->d_tile_desc) (void)hipFree(ctx->d_tile_desc);
-  if (ctx->d_tile_stage) (void)hipFree(ctx->d_tile_stage);
-  if (ctx->d_tile_local) (void)hipFree(ctx->d_tile_local);
-  ctx->d_tile_desc = ctx->d_tile_stage = nullptr;
-  ctx->d_tile_local = nullptr;
-  ctx->tiles = HostTiles();
-  ctx->tiles_built_for = 0;
-}
-
-void free_list(ljgpu_ctx* ctx) {
-  free_tiles(ctx);
-  if (ctx->d_nop) (void)hipFree(ctx->d_nop);
-  if (ctx->d_ptr) (void)hipFree(ctx->d_ptr);
-  if (ctx->d_list) (void)hipFree(ctx->d_list);
-  if (ctx->d_ell) (void)hipFree(ctx->d_ell);
-  ctx->d_nop = ctx->d_ptr = ctx->d_list = ctx->d_ell = nullptr;
-  ctx->d_list_len = 0;
-  ctx->ell_width = 0;
+// Forget the installed list; device buffers are kept for the next one.
+void reset_list(ljgpu_ctx* ctx) {
   ctx->n = 0;
   ctx->kind = -1;
   ctx->n_entries = 0;
-  ctx->eval = HostCSR();
+  ctx->max_np = ctx->min_np = 0;
+  ctx->eval.n = 0;
+  ctx->eval.e = 0;
+  ctx->eval_half = false;
+  ctx->half_mode = 0;
+  ctx->ell_ready = false;
+  ctx->ell_width = 0;
+  ctx->tiles_for = 0;
+  ctx->tiles.tile_particles = 0;
+  ctx->tiles.blocks = 0;
 }
 
-template <class T>
-int upload(ljgpu_ctx* ctx, T** dst, const std::vector<T>& src) {
-  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
-  HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), bytes), "hipMalloc (list)");
-  if (!src.empty())
-    HIP_TRY(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
-                           ctx->stream),
-            "hipMemcpyAsync (list)");
-  return LJGPU_OK;
+void release_all(ljgpu_ctx* ctx) {
+  reset_list(ctx);
+  dev_release(ctx->eval);
+  dev_release(ctx->staging);
+  dev_release(ctx->ell);
+  dev_release(ctx->tiles);
+  dev_release(ctx->work);
 }
 
-int build_ell(ljgpu_ctx* ctx) {
-  const HostCSR& c = ctx->eval;
-  int32_t width = 0;
-  for (int32_t i = 0; i < c.n; i++) width = std::max(width, c.nop[i]);
-  std::vector<int32_t> ell(static_cast<size_t>(c.n) * std::max(width, 1), 0);
-  for (int32_t i = 0; i < c.n; i++)
-    for (int32_t k = 0; k < c.nop[i]; k++)
-      ell[i + static_cast<size_t>(k) * c.n] = c.list[c.ptr[i] + k];
-  if (ctx->d_ell) (void)hipFree(ctx->d_ell);
-  ctx->d_ell = nullptr;
-  int rc = upload(ctx, &ctx->d_ell, ell);
-  if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  ctx->ell_width = std::max(width, 1);
-  return LJGPU_OK;
-}
-
-// Derive and upload the LDS tiling of the evaluated (full) list for the
-// context's tile_particles. A list whose single rows exceed the LDS staging
-// capacity keeps the untiled kernels.
 // Particles per tile for the current kernel configuration: the requested
 // tile size, capped so that a tile's workgroup (tile * lanes threads) fits
 // 1024 threads; 0 when tiling is off or the ELL kernel is selected.
@@ -134,72 +109,127 @@ int32_t effective_tile(const ljgpu_ctx* ctx) {
   return std::min(ctx->tile_particles, 1024 / ctx->lanes);
 }
 
-int build_device_tiles(ljgpu_ctx* ctx) {
-  free_tiles(ctx);
-  const int32_t tp = effective_tile(ctx);
-  if (tp <= 0 || ctx->eval_half || ctx->eval.n == 0) return LJGPU_OK;
-  HostTiles t;
-  if (!build_tiles(ctx->eval, tp, &t)) return LJGPU_OK;
-  int rc;
-  if ((rc = upload(ctx, &ctx->d_tile_desc, t.desc))) return rc;
-  if ((rc = upload(ctx, &ctx->d_tile_stage, t.stage))) return rc;
-  t.local.resize(t.local.size() + kTileLocalPad, 0);  // the kernel copies whole 16-byte chunks
-  if ((rc = upload(ctx, &ctx->d_tile_local, t.local))) return rc;
-  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  ctx->tiles = std::move(t);
-  ctx->tiles.stage.clear();
-  ctx->tiles.stage.shrink_to_fit();
-  ctx->tiles.local.clear();
-  ctx->tiles.local.shrink_to_fit();
-  ctx->tiles_built_for = tp;
+// Derive the ELL copy of the evaluated list (thread-per-particle kernel).
+int ensure_ell(ljgpu_ctx* ctx) {
+  if (ctx->ell_ready) return LJGPU_OK;
+  int32_t mn = 0, mx = 0;
+  TRY(dev_minmax(ctx->stream, ctx->eval.nop.as<int32_t>(), ctx->eval.n, ctx->work, &mn, &mx));
+  ctx->ell_width = std::max(mx, 1);
+  TRY(dev_make_ell(ctx->stream, ctx->eval, ctx->ell_width, ctx->ell));
+  ctx->ell_ready = true;
   return LJGPU_OK;
 }
 
-// Install a validated CSR list held on the host.
-int install_csr(ljgpu_ctx* ctx, int32_t n, std::vector<int32_t>&& nop, std::vector<int32_t>&& ptr,
-                std::vector<int32_t>&& list, int kind) {
-  int rc = use_device(ctx);
-  if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  free_list(ctx);
+// Derive the LDS tiling of the evaluated full list for the current
+// configuration. Where a block does not fit the LDS budget the tile size is
+// halved; a list whose single rows do not fit keeps the untiled kernels.
+int ensure_tiles(ljgpu_ctx* ctx) {
+  const int32_t want = ctx->eval_half ? 0 : effective_tile(ctx);
+  if (want == ctx->tiles_for) return LJGPU_OK;
+  ctx->tiles.tile_particles = 0;
+  ctx->tiles.blocks = 0;
+  for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
+    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->work, ctx->tiles));
+    if (ctx->tiles.tile_particles > 0) break;
+  }
+  ctx->tiles_for = want;
+  return LJGPU_OK;
+}
+
+int prepare_layouts(ljgpu_ctx* ctx) {
+  if (ctx->lanes == 1) return ensure_ell(ctx);
+  return ensure_tiles(ctx);
+}
+
+// Turn a validated device CSR (the caller's, or staged from the host) into
+// the evaluated list and its layouts.
+int derive_list(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* ptr,
+                const int32_t* list, int64_t e, int kind) {
   ctx->n = n;
   ctx->kind = kind;
-  ctx->n_entries = static_cast<int64_t>(list.size());
-  ctx->max_np = 0;
-  ctx->min_np = n > 0 ? INT32_MAX : 0;
-  for (int32_t i = 0; i < n; i++) {
-    ctx->max_np = std::max(ctx->max_np, nop[i]);
-    ctx->min_np = std::min(ctx->min_np, nop[i]);
-  }
-  HostCSR given;
-  given.n = n;
-  given.nop = std::move(nop);
-  given.ptr = std::move(ptr);
-  given.list = std::move(list);
-  if (kind == LJGPU_LIST_HALF) {
+  ctx->n_entries = e;
+  if (kind == LJGPU_LIST_HALF)
     ctx->half_mode = ctx->half_mode_req == LJGPU_HALF_ATOMIC ? LJGPU_HALF_ATOMIC : LJGPU_HALF_EXPAND;
-  } else {
+  else
     ctx->half_mode = 0;
-  }
   if (kind == LJGPU_LIST_HALF && ctx->half_mode == LJGPU_HALF_EXPAND) {
-    expand_half_to_full(given, &ctx->eval);
+    TRY(dev_expand_half(ctx->stream, n, nop, ptr, list, ctx->work, ctx->eval));
     ctx->eval_half = false;
   } else {
-    ctx->eval = std::move(given);
+    // Rows stored in order make every tile's entries one contiguous range.
+    TRY(dev_copy_row_ordered(ctx->stream, n, nop, ptr, list, e, ctx->work, ctx->eval));
     ctx->eval_half = (kind == LJGPU_LIST_HALF);
   }
-  // Rows stored in order make every tile's entries one contiguous range.
-  csr_make_row_ordered(&ctx->eval);
-  if ((rc = upload(ctx, &ctx->d_nop, ctx->eval.nop))) return rc;
-  if ((rc = upload(ctx, &ctx->d_ptr, ctx->eval.ptr))) return rc;
-  if ((rc = upload(ctx, &ctx->d_list, ctx->eval.list))) return rc;
-  ctx->d_list_len = static_cast<int64_t>(ctx->eval.list.size());
+  ctx->ell_ready = false;
+  ctx->tiles_for = -1;
+  TRY(prepare_layouts(ctx));
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  if (ctx->lanes == 1) {
-    if ((rc = build_ell(ctx))) return rc;
-  } else if (effective_tile(ctx) > 0) {
-    if ((rc = build_device_tiles(ctx))) return rc;
+  return LJGPU_OK;
+}
+
+// Install a list validated on the host.
+int install_host(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* ptr,
+                 const int32_t* list, int64_t e, int kind) {
+  TRY(use_device(ctx));
+  reset_list(ctx);
+  int32_t mn = n > 0 ? INT32_MAX : 0, mx = 0;
+  for (int32_t i = 0; i < n; i++) {
+    mn = std::min(mn, nop[i]);
+    mx = std::max(mx, nop[i]);
   }
+  DevCSR& st = ctx->staging;
+  TRY(dev_reserve(st.nop, sizeof(int32_t) * std::max(n, 1), ctx->stream));
+  TRY(dev_reserve(st.ptr, sizeof(int32_t) * std::max(n, 1), ctx->stream));
+  TRY(dev_reserve(st.list, sizeof(int32_t) * std::max<int64_t>(e, 1), ctx->stream));
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(st.nop.p, nop, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream),
+            "hipMemcpyAsync (list)");
+    HIP_TRY(hipMemcpyAsync(st.ptr.p, ptr, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream),
+            "hipMemcpyAsync (list)");
+  }
+  if (e > 0)
+    HIP_TRY(hipMemcpyAsync(st.list.p, list, sizeof(int32_t) * e, hipMemcpyHostToDevice, ctx->stream),
+            "hipMemcpyAsync (list)");
+  TRY(derive_list(ctx, n, st.nop.as<int32_t>(), st.ptr.as<int32_t>(), st.list.as<int32_t>(), e,
+                  kind));
+  ctx->min_np = mn;
+  ctx->max_np = mx;
+  return LJGPU_OK;
+}
+
+// Build the list from device positions (cell list on the device).
+int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind) {
+  if (!(search > 0.0) || n < 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "build_list: bad argument");
+  if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "build_list: bad kind");
+  if (box > 0.0 && box < 2.0 * search)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "build_list: periodic box shorter than twice the search length");
+  if (n > 0 && (!q.x || !q.y || !q.z))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "build_list: null positions");
+  TRY(use_device(ctx));
+  reset_list(ctx);
+  const bool expand = kind == LJGPU_LIST_FULL || ctx->half_mode_req != LJGPU_HALF_ATOMIC;
+  // A full list is the symmetric closure of the half list (the pair
+  // predicate is exactly symmetric), so an expanded half list is built
+  // directly as the full list.
+  TRY(dev_makepair(ctx->stream, q, n, box, search, expand ? LJGPU_LIST_FULL : LJGPU_LIST_HALF,
+                   ctx->work, ctx->eval));
+  ctx->n = n;
+  ctx->kind = kind;
+  ctx->eval_half = !expand;
+  ctx->half_mode = kind == LJGPU_LIST_HALF ? (expand ? LJGPU_HALF_EXPAND : LJGPU_HALF_ATOMIC) : 0;
+  ctx->n_entries = (kind == LJGPU_LIST_HALF && expand) ? ctx->eval.e / 2 : ctx->eval.e;
+  if (kind == LJGPU_LIST_HALF && expand)
+    TRY(dev_half_row_stats(ctx->stream, ctx->eval, ctx->work, &ctx->min_np, &ctx->max_np));
+  else
+    TRY(dev_minmax(ctx->stream, ctx->eval.nop.as<int32_t>(), n, ctx->work, &ctx->min_np,
+                   &ctx->max_np));
+  ctx->ell_ready = false;
+  ctx->tiles_for = -1;
+  TRY(prepare_layouts(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }
 
@@ -220,9 +250,9 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
   if (ctx->n == 0 || steps == 0) return LJGPU_OK;
   ListView lv;
   lv.n = ctx->n;
-  lv.nop = ctx->d_nop;
-  lv.ptr = ctx->d_ptr;
-  lv.n_entries = ctx->d_list_len;
+  lv.nop = ctx->eval.nop.as<int32_t>();
+  lv.ptr = ctx->eval.ptr.as<int32_t>();
+  lv.n_entries = ctx->eval.e;
   LaunchConfig lc;
   lc.lanes = ctx->lanes;
   lc.unroll = ctx->unroll;
@@ -232,24 +262,20 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
   lc.cus = ctx->cus;
   lc.half_atomic = ctx->eval_half;
   lc.clamp = false;
+  TRY(prepare_layouts(ctx));
   if (ctx->lanes == 1) {
-    if (!ctx->d_ell && (rc = build_ell(ctx))) return rc;
-    lv.list = ctx->d_ell;
+    lv.list = ctx->ell.as<int32_t>();
     lv.ell_width = ctx->ell_width;
   } else {
-    lv.list = ctx->d_list;
-    const int32_t tp = effective_tile(ctx);
-    if (tp > 0 && !ctx->eval_half) {
-      if (ctx->tiles_built_for != tp && (rc = build_device_tiles(ctx))) return rc;
-      if (ctx->tiles_built_for > 0) {
-        lv.tile_blocks = static_cast<int32_t>(ctx->tiles.desc.size() / 8);
-        lv.tile_particles = ctx->tiles.tile_particles;
-        lv.tile_lds_bytes = ctx->tiles.lds_bytes;
-        lv.tile_max_stage = ctx->tiles.max_stage;
-        lv.tile_desc = ctx->d_tile_desc;
-        lv.tile_stage = ctx->d_tile_stage;
-        lv.tile_local = ctx->d_tile_local;
-      }
+    lv.list = ctx->eval.list.as<int32_t>();
+    if (ctx->tiles.blocks > 0 && !ctx->eval_half) {
+      lv.tile_blocks = ctx->tiles.blocks;
+      lv.tile_particles = ctx->tiles.tile_particles;
+      lv.tile_lds_bytes = ctx->tiles.lds_bytes;
+      lv.tile_max_stage = ctx->tiles.max_stage;
+      lv.tile_desc = ctx->tiles.desc.as<int32_t>();
+      lv.tile_stage = ctx->tiles.stage.as<int32_t>();
+      lv.tile_local = ctx->tiles.local.as<uint16_t>();
     }
   }
   for (int32_t s = 0; s < steps; s++) {
@@ -306,7 +332,7 @@ int ljgpu_destroy(ljgpu_ctx* ctx) {
   if (!ctx) return LJGPU_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  free_list(ctx);
+  release_all(ctx);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -438,26 +464,24 @@ int ljgpu_set_list_csr(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int3
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "particle_number or n_entries out of range");
   if (n > 0 && (!nop || !ptr)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null list array");
   if (n_entries > 0 && !list) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null sorted_list");
-  std::vector<int32_t> h_nop(nop ? n : 0), h_ptr(ptr ? n : 0), h_list(static_cast<size_t>(n_entries));
-  if (mem == LJGPU_MEM_DEVICE) {
-    int rc = use_device(ctx);
+  if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "kind must be LJGPU_LIST_HALF or LJGPU_LIST_FULL");
+  if (mem == LJGPU_MEM_HOST) {
+    int rc = validate_csr(n, nop, ptr, list, n_entries, kind);
     if (rc) return rc;
-    if (n > 0) {
-      HIP_TRY(hipMemcpy(h_nop.data(), nop, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "hipMemcpy");
-      HIP_TRY(hipMemcpy(h_ptr.data(), ptr, sizeof(int32_t) * n, hipMemcpyDeviceToHost), "hipMemcpy");
-    }
-    if (n_entries > 0)
-      HIP_TRY(hipMemcpy(h_list.data(), list, sizeof(int32_t) * n_entries, hipMemcpyDeviceToHost), "hipMemcpy");
-  } else {
-    if (n > 0) {
-      std::memcpy(h_nop.data(), nop, sizeof(int32_t) * n);
-      std::memcpy(h_ptr.data(), ptr, sizeof(int32_t) * n);
-    }
-    if (n_entries > 0) std::memcpy(h_list.data(), list, sizeof(int32_t) * n_entries);
+    return install_host(ctx, n, nop, ptr, list, n_entries, kind);
   }
-  int rc = validate_csr(n, h_nop.data(), h_ptr.data(), h_list.data(), n_entries, kind);
-  if (rc) return rc;
-  return install_csr(ctx, n, std::move(h_nop), std::move(h_ptr), std::move(h_list), kind);
+  // Device input: validated, measured and derived on the device; the
+  // caller's buffers are only read.
+  TRY(use_device(ctx));
+  TRY(dev_validate(ctx->stream, n, nop, ptr, list, n_entries, kind, ctx->work));
+  int32_t mn = 0, mx = 0;
+  TRY(dev_minmax(ctx->stream, nop, n, ctx->work, &mn, &mx));
+  reset_list(ctx);
+  TRY(derive_list(ctx, n, nop, ptr, list, n_entries, kind));
+  ctx->min_np = mn;
+  ctx->max_np = mx;
+  return LJGPU_OK;
 }
 
 int ljgpu_set_list_transposed(ljgpu_ctx* ctx, int32_t n, const int32_t* nop,
@@ -493,7 +517,7 @@ int ljgpu_set_list_transposed(ljgpu_ctx* ctx, int32_t n, const int32_t* nop,
     for (int32_t k = 0; k < h_nop[i]; k++) h_list[h_ptr[i] + k] = h_ell[i + static_cast<size_t>(k) * n];
   int rc = validate_csr(n, h_nop.data(), h_ptr.data(), h_list.data(), pos, kind);
   if (rc) return rc;
-  return install_csr(ctx, n, std::move(h_nop), std::move(h_ptr), std::move(h_list), kind);
+  return install_host(ctx, n, h_nop.data(), h_ptr.data(), h_list.data(), pos, kind);
 }
 
 int ljgpu_get_list_info(ljgpu_ctx* ctx, ljgpu_list_info* info) {
@@ -509,25 +533,84 @@ int ljgpu_get_list_info(ljgpu_ctx* ctx, ljgpu_list_info* info) {
   info->lanes_per_particle = ctx->lanes;
   info->unroll = ctx->unroll;
   info->block_size = ctx->block;
-  info->tile_particles = ctx->tiles_built_for;
-  info->tile_blocks = static_cast<int32_t>(ctx->tiles.desc.size() / 8);
-  info->tile_max_stage = ctx->tiles.max_stage;
+  const bool tiled = ctx->lanes > 1 && ctx->tiles.blocks > 0 && !ctx->eval_half &&
+                     ctx->tiles_for == effective_tile(ctx);
+  info->tile_particles = tiled ? ctx->tiles.tile_particles : 0;
+  info->tile_blocks = tiled ? ctx->tiles.blocks : 0;
+  info->tile_max_stage = tiled ? ctx->tiles.max_stage : 0;
   return LJGPU_OK;
 }
 
 int ljgpu_get_list_csr(ljgpu_ctx* ctx, int32_t* nop, int32_t* ptr, int32_t* list, int64_t capacity) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
-  const HostCSR& c = ctx->eval;
-  if (static_cast<int64_t>(c.list.size()) > capacity) return set_error(LJGPU_ERR_CAPACITY, "capacity too small");
+  const DevCSR& c = ctx->eval;
+  if (c.e > capacity) return set_error(LJGPU_ERR_CAPACITY, "capacity too small");
   if (c.n > 0 && (!nop || !ptr)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null output");
-  if (!c.list.empty() && !list) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null output");
+  if (c.e > 0 && !list) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null output");
+  TRY(use_device(ctx));
   if (c.n > 0) {
-    std::memcpy(nop, c.nop.data(), sizeof(int32_t) * c.n);
-    std::memcpy(ptr, c.ptr.data(), sizeof(int32_t) * c.n);
+    HIP_TRY(hipMemcpyAsync(nop, c.nop.p, sizeof(int32_t) * c.n, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(ptr, c.ptr.p, sizeof(int32_t) * c.n, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpyAsync");
   }
-  if (!c.list.empty()) std::memcpy(list, c.list.data(), sizeof(int32_t) * c.list.size());
+  if (c.e > 0)
+    HIP_TRY(hipMemcpyAsync(list, c.list.p, sizeof(int32_t) * c.e, hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
+}
+
+int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_t* stage,
+                    int64_t stage_capacity, uint16_t* local, int64_t local_capacity,
+                    int32_t* n_blocks, int64_t* n_stage) {
+  if (!ctx || !n_blocks || !n_stage) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  TRY(use_device(ctx));
+  TRY(prepare_layouts(ctx));
+  const DevTiles& t = ctx->tiles;
+  const bool tiled = ctx->lanes > 1 && t.blocks > 0 && !ctx->eval_half;
+  *n_blocks = tiled ? t.blocks : 0;
+  *n_stage = tiled ? t.staged : 0;
+  if (!tiled || (!desc && !stage && !local)) return LJGPU_OK;
+  if (!desc || !stage || !local || desc_capacity < 8 * static_cast<int64_t>(t.blocks) ||
+      stage_capacity < t.staged || local_capacity < ctx->eval.e)
+    return set_error(LJGPU_ERR_CAPACITY, "get_tiles: output too small");
+  HIP_TRY(hipMemcpyAsync(desc, t.desc.p, sizeof(int32_t) * 8 * t.blocks, hipMemcpyDeviceToHost,
+                         ctx->stream), "hipMemcpyAsync");
+  if (t.staged > 0)
+    HIP_TRY(hipMemcpyAsync(stage, t.stage.p, sizeof(int32_t) * t.staged, hipMemcpyDeviceToHost,
+                           ctx->stream), "hipMemcpyAsync");
+  if (ctx->eval.e > 0)
+    HIP_TRY(hipMemcpyAsync(local, t.local.p, sizeof(uint16_t) * ctx->eval.e, hipMemcpyDeviceToHost,
+                           ctx->stream), "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return LJGPU_OK;
+}
+
+int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t n, double box,
+                     double search_length, int kind) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (n > 0 && (reinterpret_cast<uintptr_t>(q) & 15u))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q must be 16-byte aligned");
+  PosIn in;
+  in.x = q;
+  in.y = q ? q + 1 : nullptr;
+  in.z = q ? q + 2 : nullptr;
+  in.stride = 4;
+  return build_list(ctx, in, n, box, search_length, kind);
+}
+
+int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
+                         int32_t n, double box, double search_length, int kind) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  PosIn in;
+  in.x = qx;
+  in.y = qy;
+  in.z = qz;
+  in.stride = 1;
+  return build_list(ctx, in, n, box, search_length, kind);
 }
 
 int ljgpu_force_aos_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,

@@ -171,6 +171,34 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out) {
   return true;
 }
 
+void make_cell_grid(int32_t n, const double lo[3], const double hi[3], double box,
+                    double search_length, CellGrid* g) {
+  g->brute = false;
+  for (int c = 0; c < 3; c++) {
+    const double extent = box > 0.0 ? box : (hi[c] - lo[c]);
+    g->origin[c] = box > 0.0 ? 0.0 : lo[c];
+    g->extent[c] = extent;
+    // Cells are kept a hair wider than the search length so that rounding
+    // in x / width can never put two particles closer than the search
+    // length two cells apart.
+    int m = static_cast<int>(extent / (search_length * (1.0 + 1e-9)));
+    g->nc[c] = std::min(std::max(m, 1), 1024);
+  }
+  // At most a few cells per particle: coarser cells stay correct (wider than
+  // the search length) and keep the cell arrays proportional to N.
+  const int64_t limit = 8 * static_cast<int64_t>(std::max(n, 1)) + 64;
+  while (static_cast<int64_t>(g->nc[0]) * g->nc[1] * g->nc[2] > limit) {
+    int* m = std::max_element(g->nc, g->nc + 3);
+    *m = (*m + 1) / 2;
+  }
+  for (int c = 0; c < 3; c++) {
+    g->width[c] = g->extent[c] > 0.0 ? g->extent[c] / g->nc[c] : 1.0;
+    if (box > 0.0 && g->nc[c] < 3) g->brute = true;
+  }
+  if (g->brute) g->nc[0] = g->nc[1] = g->nc[2] = 1;
+  g->ncell = static_cast<int64_t>(g->nc[0]) * g->nc[1] * g->nc[2];
+}
+
 }  // namespace ljgpu
 
 using namespace ljgpu;
@@ -289,19 +317,12 @@ int ljgpu_makepair_host(int32_t n, const double* q, double box, double search_le
       lo[c] = std::min(lo[c], q[4 * (int64_t)i + c]);
       hi[c] = std::max(hi[c], q[4 * (int64_t)i + c]);
     }
-  int nc[3];
-  double origin[3], width[3];
-  bool brute = false;
-  for (int c = 0; c < 3; c++) {
-    const double extent = box > 0.0 ? box : (hi[c] - lo[c]);
-    origin[c] = box > 0.0 ? 0.0 : lo[c];
-    int m = static_cast<int>(extent / search_length);
-    if (m < 1) m = 1;
-    if (m > 1024) m = 1024;
-    nc[c] = m;
-    width[c] = extent > 0.0 ? extent / m : 1.0;
-    if (box > 0.0 && m < 3) brute = true;
-  }
+  CellGrid g;
+  make_cell_grid(n, lo, hi, box, search_length, &g);
+  const int* nc = g.nc;
+  const double* origin = g.origin;
+  const double* width = g.width;
+  const bool brute = g.brute;
   const int64_t ncell = static_cast<int64_t>(nc[0]) * nc[1] * nc[2];
   std::vector<int32_t> cell_of(n), cell_start(ncell + 1, 0), cell_items(n);
   for (int32_t i = 0; i < n; i++) {

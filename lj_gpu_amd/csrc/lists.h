@@ -1,0 +1,120 @@
+// lists.h — neighbour lists on the device: the Verlet-list build from device
+// positions (cell list, the reference's exact pair set and CSR order) and
+// every layout the force kernels evaluate, derived on the device from a CSR
+// list (half-to-full expansion, row ordering, ELL, LDS tiles). HIP-only
+// header, used by api.hip and lists.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "ljgpu_internal.h"
+
+namespace ljgpu {
+
+// A device allocation that only grows: reserve() reallocates (after
+// synchronising `stream`) when more bytes are needed, so repeated list
+// builds of similar size reuse their buffers.
+struct DevMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+int dev_reserve(DevMem& m, size_t bytes, hipStream_t stream);
+void dev_release(DevMem& m);
+
+// A CSR list on the device: number_of_partners[n], pointer[n + 1] (pointer[n]
+// = e when rows are stored in order), sorted_list[e].
+struct DevCSR {
+  int32_t n = 0;
+  int64_t e = 0;
+  DevMem nop, ptr, list;
+};
+void dev_release(DevCSR& c);
+
+// LDS tiling of a row-ordered full list (the layout ListView describes).
+struct DevTiles {
+  int32_t tile_particles = 0;  // particles per block (0: no tiling)
+  int32_t blocks = 0;
+  int64_t staged = 0;
+  int32_t max_stage = 0;
+  int32_t lds_bytes = 0;
+  DevMem desc, stage, local;
+};
+void dev_release(DevTiles& t);
+
+// Scratch buffers reused across builds.
+struct ListWork {
+  DevMem scan_sums;   // int64 per scan tile
+  DevMem small;       // a few int64 results (totals, flags, extrema)
+  DevMem cell_count;  // int32 per cell (+1)
+  DevMem cell_start;  // int32 per cell (+1)
+  DevMem cell_slot;   // int32 per particle: rank within its cell / fill counters
+  DevMem cell_of;     // int32 per particle: cell index
+  DevMem cell_sorted; // int32 per particle, cell order: cell index
+  DevMem cpos;        // double4 per particle, cell order: x, y, z, index
+  DevMem counts;      // int32 per row (+1)
+  DevMem tmp;         // int32 per list entry
+  DevMem tile_ns;     // int32 per tile (+1)
+  DevMem tile_soff;   // int32 per tile (+1)
+};
+void dev_release(ListWork& w);
+
+// Positions as three strided arrays (AoS: q, q+1, q+2 with stride 4; SoA:
+// three arrays with stride 1).
+struct PosIn {
+  const double* x = nullptr;
+  const double* y = nullptr;
+  const double* z = nullptr;
+  int stride = 4;
+};
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total; *total receives
+// the sum (synchronises the stream).
+int dev_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, ListWork& w,
+             int64_t* total);
+
+// Strict check of a CSR list on the device (the rules of validate_csr);
+// returns LJGPU_OK or LJGPU_ERR_BAD_LIST naming the first bad row.
+int dev_validate(hipStream_t s, int32_t n, const int32_t* nop, const int32_t* ptr,
+                 const int32_t* list, int64_t e, int kind, ListWork& w);
+
+// Copy a caller's device CSR into `out` with rows stored in order.
+int dev_copy_row_ordered(hipStream_t s, int32_t n, const int32_t* nop, const int32_t* ptr,
+                         const int32_t* list, int64_t e, ListWork& w, DevCSR& out);
+
+// Full list from a half list (both on the device): row i holds its lower
+// partners ascending, then its own half row as stored.
+int dev_expand_half(hipStream_t s, int32_t n, const int32_t* nop, const int32_t* ptr,
+                    const int32_t* list, ListWork& w, DevCSR& full);
+
+// Verlet list of `kind` from device positions: pair predicate
+// r2 = (dx*dx + dy*dy) + dz*dz < search^2, unfused, minimum image for
+// box > 0; rows ascending in i, partners ascending.
+int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double search_length,
+                 int kind, ListWork& w, DevCSR& out);
+
+// Smallest and largest of a[0..n) (synchronises); 0, 0 for n = 0.
+int dev_minmax(hipStream_t s, const int32_t* a, int32_t n, ListWork& w, int32_t* mn, int32_t* mx);
+// Row lengths of the half list contained in a full list with ascending rows
+// (partners j > i), as min/max (synchronises).
+int dev_half_row_stats(hipStream_t s, const DevCSR& full, ListWork& w, int32_t* min_np,
+                       int32_t* max_np);
+
+// ELL ("transposed") copy, entry k of row i at ell[i + k*n], width = longest
+// row (at least 1), zero padded.
+int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell);
+
+// LDS tiling of a row-ordered full list with blocks of `tile_particles`
+// consecutive particles. Returns LJGPU_OK with out.tile_particles = 0 when a
+// block does not fit the LDS budget (the caller then halves the tile size or
+// evaluates untiled).
+int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles, ListWork& w,
+                   DevTiles& out);
+
+}  // namespace ljgpu

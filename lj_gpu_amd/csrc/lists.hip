@@ -1,0 +1,943 @@
+// lists.hip — neighbour lists on the device (see lists.h).
+//
+// The Verlet-list build follows the reference's makepair()/sortpair()
+// (cpu_ref/force_aos.cpp:100-140; full list cuda/force_cuda.cu:122-163): the
+// same pair predicate, evaluated unfused so every pair decision is
+// bit-identical to the reference's, and the same CSR (rows ascending in i,
+// partners ascending in j). The O(N^2) pair scan is replaced by a cell list:
+//   1. bin particles into cells at least the search length wide;
+//   2. count each row's partners over the 27 neighbouring cells, scan the
+//      counts into row pointers;
+//   3. write each row's partners, then sort every row.
+// Derived layouts (full list from a half list, ELL, LDS tiles) are built here
+// from the device CSR, so a list never round-trips through the host.
+#include "lists.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace ljgpu {
+
+namespace {
+
+int hip_err(hipError_t e, const char* what) {
+  return set_error(e == hipErrorOutOfMemory ? LJGPU_ERR_OUT_OF_MEMORY : LJGPU_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define LTRY(expr, what)                          \
+  do {                                            \
+    hipError_t e_ = (expr);                       \
+    if (e_ != hipSuccess) return hip_err(e_, what); \
+  } while (0)
+
+#define LCHECK(call)        \
+  do {                      \
+    int rc_ = (call);       \
+    if (rc_) return rc_;    \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+int grid_for(int64_t items, int block, int cap = 65535 * 8) {
+  const int64_t g = (items + block - 1) / block;
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g, cap)));
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill(T* __restrict__ a, int64_t n, T v) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[t] = v;
+}
+
+template <class T>
+int fill(hipStream_t s, T* a, int64_t n, T v) {
+  if (n <= 0) return LJGPU_OK;
+  k_fill<T><<<grid_for(n, kBlock), kBlock, 0, s>>>(a, n, v);
+  LTRY(hipGetLastError(), "fill kernel");
+  return LJGPU_OK;
+}
+
+// Inclusive scan of one value per lane over a wavefront.
+__device__ __forceinline__ int64_t wave_inclusive(int64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over a kBlock-thread workgroup;
+// *total receives the workgroup sum. `sh` holds kBlock/64 values.
+__device__ __forceinline__ int64_t block_exclusive(int64_t v, int64_t* total, int64_t* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t inc = wave_inclusive(v);
+  if (lane == 63) sh[wid] = inc;
+  __syncthreads();
+  int64_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; w++) {
+    const int64_t x = sh[w];
+    off += (w < wid) ? x : 0;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// Scan pass 1: per-tile sums.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                        int64_t* __restrict__ sums) {
+  __shared__ int64_t sh[kBlock / 64];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int64_t v = 0;
+  for (int r = 0; r < kScanItems; r++) {
+    const int64_t t = base + r * kBlock + threadIdx.x;
+    if (t < n) v += in[t];
+  }
+  int64_t total;
+  (void)block_exclusive(v, &total, sh);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Scan pass 2: exclusive scan of the tile sums in place (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_scan_sums(int64_t* __restrict__ sums, int64_t nb,
+                                                      int64_t* __restrict__ total_out) {
+  __shared__ int64_t sh[kBlock / 64];
+  int64_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += kBlock) {
+    const int64_t t = b0 + threadIdx.x;
+    const int64_t v = t < nb ? sums[t] : 0;
+    int64_t tot;
+    const int64_t ex = block_exclusive(v, &tot, sh);
+    if (t < nb) sums[t] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+// Scan pass 3: exclusive scan within each tile plus the tile offset; the
+// last tile also writes out[n] = total (clamped to int32; the host rejects
+// totals above INT32_MAX).
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const int32_t* __restrict__ in, int64_t n,
+                                                       const int64_t* __restrict__ sums,
+                                                       const int64_t* __restrict__ total,
+                                                       int32_t* __restrict__ out) {
+  __shared__ int64_t sh[kBlock / 64];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int64_t carry = sums[blockIdx.x];
+  for (int r = 0; r < kScanItems; r++) {
+    const int64_t t = base + r * kBlock + threadIdx.x;
+    const int64_t v = t < n ? in[t] : 0;
+    int64_t tot;
+    const int64_t ex = block_exclusive(v, &tot, sh);
+    if (t < n) out[t] = static_cast<int32_t>(std::min<int64_t>(carry + ex, INT32_MAX));
+    carry += tot;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    out[n] = static_cast<int32_t>(std::min<int64_t>(*total, INT32_MAX));
+}
+
+// ---------------------------------------------------------------------------
+// Validation (validate_csr's rules; reason codes name the defect)
+// ---------------------------------------------------------------------------
+enum : int { kBadCount = 1, kBadPointer = 2, kBadIndex = 3, kSelf = 4, kHalfOrder = 5 };
+
+__global__ void k_validate(int32_t n, const int32_t* __restrict__ nop,
+                           const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                           int64_t e, int half, unsigned long long* __restrict__ bad) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = static_cast<int32_t>(t);
+    const int64_t np = nop[i], kp = ptr[i];
+    int reason = 0;
+    if (np < 0 || np >= (n > 1 ? n : 1)) reason = kBadCount;
+    else if (kp < 0 || kp + np > e) reason = kBadPointer;
+    else
+      for (int64_t k = 0; k < np; k++) {
+        const int32_t j = list[kp + k];
+        if (j < 0 || j >= n) { reason = kBadIndex; break; }
+        if (j == i) { reason = kSelf; break; }
+        if (half && j < i) { reason = kHalfOrder; break; }
+      }
+    if (reason) atomicMin(bad, static_cast<unsigned long long>(i) * 8ull + reason);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row operations
+// ---------------------------------------------------------------------------
+// Copy row i of (ptr, list) to dst[dptr[i] ..): one wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_copy_rows(int32_t n, const int32_t* __restrict__ nop,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ dptr,
+                                                      int32_t* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6); r < n;
+       r += waves) {
+    const int32_t m = nop[r];
+    const int64_t s = ptr[r], d = dptr[r];
+    for (int32_t k = lane; k < m; k += 64) dst[d + k] = list[s + k];
+  }
+}
+
+// Stable sort of each segment [start[r], start[r] + len[r]) of src into the
+// same positions of dst, by rank: element a goes to the number of elements
+// smaller than it plus the equal ones before it. One wavefront per segment,
+// the segment staged in LDS when it fits.
+constexpr int kSortLds = 2048;
+
+__global__ __launch_bounds__(64) void k_row_sort(int32_t n, const int32_t* __restrict__ len,
+                                                 const int32_t* __restrict__ start,
+                                                 const int32_t* __restrict__ src,
+                                                 int32_t* __restrict__ dst) {
+  __shared__ int32_t sh[kSortLds];
+  const int lane = threadIdx.x;
+  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
+    const int32_t m = len[r];
+    const int64_t s = start[r];
+    if (m <= 0) continue;
+    if (m <= kSortLds) {
+      for (int32_t k = lane; k < m; k += 64) sh[k] = src[s + k];
+      __syncthreads();
+      for (int32_t a = lane; a < m; a += 64) {
+        const int32_t x = sh[a];
+        int32_t rank = 0;
+        for (int32_t b = 0; b < m; b++) {
+          const int32_t y = sh[b];
+          rank += (y < x) | ((y == x) & (b < a));
+        }
+        dst[s + rank] = x;
+      }
+      __syncthreads();
+    } else {
+      for (int32_t a = lane; a < m; a += 64) {
+        const int32_t x = src[s + a];
+        int32_t rank = 0;
+        for (int32_t b = 0; b < m; b++) {
+          const int32_t y = src[s + b];
+          rank += (y < x) | ((y == x) & (b < a));
+        }
+        dst[s + rank] = x;
+      }
+    }
+  }
+}
+
+int row_sort(hipStream_t s, int32_t n, const int32_t* len, const int32_t* start,
+             const int32_t* src, int32_t* dst) {
+  if (n <= 0) return LJGPU_OK;
+  const int grid = static_cast<int>(std::min<int64_t>(n, 1 << 20));
+  k_row_sort<<<grid, 64, 0, s>>>(n, len, start, src, dst);
+  LTRY(hipGetLastError(), "row sort kernel");
+  return LJGPU_OK;
+}
+
+// Half-to-full expansion.
+__global__ void k_count_lower(int32_t n, const int32_t* __restrict__ nop,
+                              const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                              int32_t* __restrict__ lower) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t kp = ptr[t];
+    for (int32_t k = 0; k < nop[t]; k++) atomicAdd(&lower[list[kp + k]], 1);
+  }
+}
+
+__global__ void k_add(int32_t n, const int32_t* __restrict__ a, const int32_t* __restrict__ b,
+                      int32_t* __restrict__ c) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    c[t] = a[t] + b[t];
+}
+
+__global__ void k_fill_expanded(int32_t n, const int32_t* __restrict__ nop,
+                                const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                                const int32_t* __restrict__ lower,
+                                const int32_t* __restrict__ fptr, int32_t* __restrict__ fill,
+                                int32_t* __restrict__ lower_tmp, int32_t* __restrict__ flist) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t r = static_cast<int32_t>(t);
+    const int64_t kp = ptr[r];
+    const int64_t own = static_cast<int64_t>(fptr[r]) + lower[r];
+    for (int32_t k = 0; k < nop[r]; k++) {
+      const int32_t j = list[kp + k];
+      const int32_t pos = atomicAdd(&fill[j], 1);
+      lower_tmp[static_cast<int64_t>(fptr[j]) + pos] = r;
+      flist[own + k] = j;
+    }
+  }
+}
+
+// Row-length extrema (int32 atomics on small[0] = min, small[1] = max).
+__global__ void k_minmax(int32_t n, const int32_t* __restrict__ a, int32_t* __restrict__ mm) {
+  int32_t lo = INT32_MAX, hi = INT32_MIN;
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    lo = min(lo, a[t]);
+    hi = max(hi, a[t]);
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d, 64));
+    hi = max(hi, __shfl_xor(hi, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&mm[0], lo);
+    atomicMax(&mm[1], hi);
+  }
+}
+
+// Partners j > i in each ascending full row.
+__global__ void k_upper_counts(int32_t n, const int32_t* __restrict__ nop,
+                               const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                               int32_t* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = static_cast<int32_t>(t);
+    const int32_t* row = list + ptr[i];
+    int32_t lo = 0, hi = nop[i];
+    while (lo < hi) {  // first position with row[pos] > i
+      const int32_t mid = (lo + hi) >> 1;
+      if (row[mid] <= i) lo = mid + 1;
+      else hi = mid;
+    }
+    out[i] = nop[i] - lo;
+  }
+}
+
+__global__ void k_ell(int32_t n, int32_t width, const int32_t* __restrict__ nop,
+                      const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                      int32_t* __restrict__ ell) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t kp = ptr[t];
+    const int32_t m = nop[t];
+    for (int32_t k = 0; k < width; k++) ell[t + static_cast<int64_t>(k) * n] = k < m ? list[kp + k] : 0;
+  }
+}
+
+// Minimum and maximum of a strided coordinate stream (one workgroup).
+__global__ __launch_bounds__(kBlock) void k_extrema(int32_t n, const double* __restrict__ a,
+                                                    int stride, double* __restrict__ out) {
+  __shared__ double smin[kBlock], smax[kBlock];
+  double mn = INFINITY, mx = -INFINITY;
+  for (int64_t t = threadIdx.x; t < n; t += kBlock) {
+    const double v = a[t * stride];
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + d]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + d]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smin[0];
+    out[1] = smax[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell list and pair passes
+// ---------------------------------------------------------------------------
+struct GridArgs {
+  int nc0, nc1, nc2;
+  double o0, o1, o2;
+  double w0, w1, w2;
+  double box;
+};
+
+__device__ __forceinline__ int cell_coord(double x, double origin, double width, int m, double box) {
+  x -= origin;
+  if (box > 0.0) x -= box * floor(x / box);  // the wrapped image's cell
+  const int v = static_cast<int>(floor(x / width));
+  return min(max(v, 0), m - 1);
+}
+
+__global__ void k_cell_assign(int32_t n, PosIn q, GridArgs g, int32_t* __restrict__ count,
+                              int32_t* __restrict__ slot, int32_t* __restrict__ cell_of) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t o = t * q.stride;
+    const int cx = cell_coord(q.x[o], g.o0, g.w0, g.nc0, g.box);
+    const int cy = cell_coord(q.y[o], g.o1, g.w1, g.nc1, g.box);
+    const int cz = cell_coord(q.z[o], g.o2, g.w2, g.nc2, g.box);
+    const int32_t c = (cz * g.nc1 + cy) * g.nc0 + cx;
+    cell_of[t] = c;
+    slot[t] = atomicAdd(&count[c], 1);
+  }
+}
+
+__global__ void k_cell_scatter(int32_t n, PosIn q, const int32_t* __restrict__ cell_of,
+                               const int32_t* __restrict__ slot,
+                               const int32_t* __restrict__ cell_start,
+                               double4* __restrict__ cpos, int32_t* __restrict__ cell_sorted) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t o = t * q.stride;
+    const int32_t c = cell_of[t];
+    const int64_t d = static_cast<int64_t>(cell_start[c]) + slot[t];
+    cpos[d] = make_double4(q.x[o], q.y[o], q.z[o], static_cast<double>(t));
+    cell_sorted[d] = c;
+  }
+}
+
+// The reference's pair predicate (cpu_ref/force_aos.cpp:131-135) with the
+// minimum image of the periodic box: every operation rounded separately, in
+// the reference's order, (dx*dx + dy*dy) + dz*dz < SL2.
+__device__ __forceinline__ bool within_search(const double4& a, const double4& b, double box,
+                                              double half_box, double sl2) {
+#pragma clang fp contract(off)
+  double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  if (box > 0.0) {
+    if (dx > half_box) dx -= box;
+    else if (dx < -half_box) dx += box;
+    if (dy > half_box) dy -= box;
+    else if (dy < -half_box) dy += box;
+    if (dz > half_box) dz -= box;
+    else if (dz < -half_box) dz += box;
+  }
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  return r2 < sl2;
+}
+
+// One thread per particle in cell order (lanes of a wavefront share their
+// home cell, so candidate loads are mostly broadcasts). FILL = false counts
+// each row's partners into nop[i]; FILL = true writes them to
+// out[ptr[i] ..) in candidate order.
+template <bool FILL, bool HALF>
+__global__ __launch_bounds__(kBlock) void k_pairs(int32_t n, const double4* __restrict__ cpos,
+                                                  const int32_t* __restrict__ cell_sorted,
+                                                  const int32_t* __restrict__ cell_start,
+                                                  GridArgs g, int stencil, double sl2,
+                                                  int32_t* __restrict__ nop,
+                                                  const int32_t* __restrict__ ptr,
+                                                  int32_t* __restrict__ out) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  const double4 pi = cpos[t];
+  const int32_t i = static_cast<int32_t>(pi.w);
+  const int32_t c = cell_sorted[t];
+  const int cx = c % g.nc0, cy = (c / g.nc0) % g.nc1, cz = c / (g.nc0 * g.nc1);
+  const bool periodic = g.box > 0.0;
+  const double half_box = 0.5 * g.box;
+  const int lo = stencil == 3 ? -1 : 0, hi = stencil == 3 ? 1 : 0;
+  int64_t base = 0;
+  if (FILL) base = ptr[i];
+  int32_t cnt = 0;
+  for (int dz = lo; dz <= hi; dz++) {
+    int z = cz + dz;
+    if (periodic) z = (z + g.nc2) % g.nc2;
+    else if (z < 0 || z >= g.nc2) continue;
+    for (int dy = lo; dy <= hi; dy++) {
+      int y = cy + dy;
+      if (periodic) y = (y + g.nc1) % g.nc1;
+      else if (y < 0 || y >= g.nc1) continue;
+      for (int dx = lo; dx <= hi; dx++) {
+        int x = cx + dx;
+        if (periodic) x = (x + g.nc0) % g.nc0;
+        else if (x < 0 || x >= g.nc0) continue;
+        const int32_t cc = (z * g.nc1 + y) * g.nc0 + x;
+        const int32_t u1 = cell_start[cc + 1];
+        for (int32_t u = cell_start[cc]; u < u1; u++) {
+          const double4 pj = cpos[u];
+          const int32_t j = static_cast<int32_t>(pj.w);
+          if (HALF ? (j <= i) : (j == i)) continue;
+          if (within_search(pi, pj, g.box, half_box, sl2)) {
+            if (FILL) out[base + cnt] = j;
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  if (!FILL) nop[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// LDS tiles
+// ---------------------------------------------------------------------------
+constexpr int kTileHash = 8192;                        // open-addressing table slots
+constexpr int kTileHashLimit = 4096;                   // distinct keys before giving up
+constexpr int kTileMaxStage = kTileLdsBudget / 24 + 1; // staged positions that can fit
+static_assert(kTileMaxStage <= kTileHashLimit, "the hash limit must cover every tile that fits");
+
+// Distinct staged particles of a block: its own particles and every entry's
+// partner. Keys go into an LDS hash set, are compacted into uniq[] and
+// sorted ascending there. Returns the number of distinct keys, or -1 when
+// there are more than kTileHashLimit (the block cannot fit the LDS budget).
+__device__ int32_t tile_unique(int32_t i0, int32_t ni, const int32_t* __restrict__ list, int64_t e0,
+                               int32_t ne, int32_t* table, int32_t* uniq, int32_t* counter,
+                               int64_t* sh) {
+  for (int32_t t = threadIdx.x; t < kTileHash; t += kBlock) table[t] = -1;
+  if (threadIdx.x == 0) *counter = 0;
+  __syncthreads();
+  const int64_t m = ni + static_cast<int64_t>(ne);
+  for (int64_t t = threadIdx.x; t < m; t += kBlock) {
+    const int32_t key = t < ni ? i0 + static_cast<int32_t>(t) : list[e0 + (t - ni)];
+    uint32_t h = (static_cast<uint32_t>(key) * 2654435761u) >> (32 - 13);
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <=
+           kTileHashLimit) {
+      const int32_t prev = atomicCAS(&table[h], -1, key);
+      if (prev == -1) {
+        atomicAdd(counter, 1);
+        break;
+      }
+      if (prev == key) break;
+      h = (h + 1) & (kTileHash - 1);
+    }
+  }
+  __syncthreads();
+  const int32_t ns = *counter;
+  if (ns > kTileHashLimit) {
+    __syncthreads();
+    return -1;
+  }
+  // Compact the occupied slots (32 per thread) into uniq[0..ns).
+  constexpr int32_t C = kTileHash / kBlock;
+  int32_t cnt = 0;
+  for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++) cnt += table[t] != -1;
+  int64_t total;
+  int64_t off = block_exclusive(cnt, &total, sh);
+  for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++)
+    if (table[t] != -1) uniq[off++] = table[t];
+  int32_t P = 1;
+  while (P < ns) P <<= 1;
+  for (int32_t t = ns + threadIdx.x; t < P; t += kBlock) uniq[t] = INT32_MAX;
+  __syncthreads();
+  for (int32_t k = 2; k <= P; k <<= 1)
+    for (int32_t j = k >> 1; j > 0; j >>= 1) {
+      for (int32_t t = threadIdx.x; t < P / 2; t += kBlock) {
+        const int32_t a = (t / j) * 2 * j + (t % j), b = a + j;
+        const bool up = (a & k) == 0;
+        const int32_t x = uniq[a], y = uniq[b];
+        if ((x > y) == up) {
+          uniq[a] = y;
+          uniq[b] = x;
+        }
+      }
+      __syncthreads();
+    }
+  return ns;
+}
+
+struct TileGeom {
+  int32_t i0, ni, ne;
+  int64_t e0;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(int32_t b, int32_t T, int32_t n,
+                                              const int32_t* __restrict__ nop,
+                                              const int32_t* __restrict__ ptr) {
+  TileGeom g;
+  g.i0 = b * T;
+  g.ni = min(T, n - g.i0);
+  g.e0 = ptr[g.i0];
+  g.ne = static_cast<int32_t>(static_cast<int64_t>(ptr[g.i0 + g.ni - 1]) + nop[g.i0 + g.ni - 1] - g.e0);
+  return g;
+}
+
+// Pass 1: distinct staged particles per block; flags blocks that exceed the
+// LDS budget. stats: [0] overflow, [1] max staged, [2] max LDS bytes.
+__global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int32_t nblocks,
+                                                       const int32_t* __restrict__ nop,
+                                                       const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ list,
+                                                       int32_t* __restrict__ ns_out,
+                                                       int32_t* __restrict__ stats) {
+  __shared__ int32_t table[kTileHash];
+  __shared__ int32_t uniq[kTileHashLimit];
+  __shared__ int32_t counter;
+  __shared__ int64_t sh[kBlock / 64];
+  for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const TileGeom g = tile_geom(b, T, n, nop, ptr);
+    const int32_t ns = tile_unique(g.i0, g.ni, list, g.e0, g.ne, table, uniq, &counter, sh);
+    if (threadIdx.x == 0) {
+      ns_out[b] = ns < 0 ? 0 : ns;
+      const int64_t bytes = ns < 0 ? INT32_MAX : tile_lds_bytes(ns, g.ne);
+      if (bytes > kTileLdsBudget) atomicOr(&stats[0], 1);
+      atomicMax(&stats[1], ns);
+      atomicMax(&stats[2], static_cast<int32_t>(std::min<int64_t>(bytes, INT32_MAX)));
+    }
+    __syncthreads();
+  }
+}
+
+// Pass 2: descriptors, staged indices and per-entry LDS offsets.
+__global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int32_t nblocks,
+                                                      const int32_t* __restrict__ nop,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ soff,
+                                                      int32_t* __restrict__ desc,
+                                                      int32_t* __restrict__ stage,
+                                                      uint16_t* __restrict__ local) {
+  __shared__ int32_t table[kTileHash];
+  __shared__ int32_t uniq[kTileHashLimit];
+  __shared__ int32_t counter;
+  __shared__ int64_t sh[kBlock / 64];
+  for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const TileGeom g = tile_geom(b, T, n, nop, ptr);
+    const int32_t ns = tile_unique(g.i0, g.ni, list, g.e0, g.ne, table, uniq, &counter, sh);
+    const int64_t s0 = soff[b];
+    for (int32_t t = threadIdx.x; t < ns; t += kBlock) stage[s0 + t] = uniq[t];
+    if (threadIdx.x == 0) {
+      int32_t* d = desc + static_cast<int64_t>(b) * 8;
+      d[0] = g.i0;
+      d[1] = g.ni;
+      d[2] = static_cast<int32_t>(s0);
+      d[3] = ns;
+      d[4] = static_cast<int32_t>(g.e0);
+      d[5] = g.ne;
+      d[6] = 0;
+      d[7] = 0;
+    }
+    for (int32_t e = threadIdx.x; e < g.ne; e += kBlock) {
+      const int32_t j = list[g.e0 + e];
+      int32_t lo = 0, hi = ns;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (uniq[mid] < j) lo = mid + 1;
+        else hi = mid;
+      }
+      local[g.e0 + e] = static_cast<uint16_t>(lo * 24);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int dev_reserve(DevMem& m, size_t bytes, hipStream_t stream) {
+  bytes = std::max<size_t>(bytes, 16);
+  if (m.p && m.bytes >= bytes) return LJGPU_OK;
+  if (m.p) {
+    LTRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    (void)hipFree(m.p);
+    m.p = nullptr;
+    m.bytes = 0;
+  }
+  const size_t grow = bytes + bytes / 8;  // headroom for the next, slightly larger build
+  LTRY(hipMalloc(&m.p, grow), "hipMalloc (list buffer)");
+  m.bytes = grow;
+  return LJGPU_OK;
+}
+
+void dev_release(DevMem& m) {
+  if (m.p) (void)hipFree(m.p);
+  m.p = nullptr;
+  m.bytes = 0;
+}
+
+void dev_release(DevCSR& c) {
+  dev_release(c.nop);
+  dev_release(c.ptr);
+  dev_release(c.list);
+  c.n = 0;
+  c.e = 0;
+}
+
+void dev_release(DevTiles& t) {
+  dev_release(t.desc);
+  dev_release(t.stage);
+  dev_release(t.local);
+  t.tile_particles = 0;
+  t.blocks = 0;
+  t.staged = 0;
+  t.max_stage = 0;
+  t.lds_bytes = 0;
+}
+
+void dev_release(ListWork& w) {
+  for (DevMem* m : {&w.scan_sums, &w.small, &w.cell_count, &w.cell_start, &w.cell_slot, &w.cell_of,
+                    &w.cell_sorted, &w.cpos, &w.counts, &w.tmp, &w.tile_ns, &w.tile_soff})
+    dev_release(*m);
+}
+
+int dev_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, ListWork& w,
+             int64_t* total) {
+  *total = 0;
+  if (n <= 0) {
+    LCHECK(fill<int32_t>(s, out, 1, 0));
+    return LJGPU_OK;
+  }
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  LCHECK(dev_reserve(w.scan_sums, sizeof(int64_t) * nb, s));
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  int64_t* sums = w.scan_sums.as<int64_t>();
+  int64_t* tot = w.small.as<int64_t>();
+  k_scan_reduce<<<static_cast<int>(nb), kBlock, 0, s>>>(in, n, sums);
+  k_scan_sums<<<1, kBlock, 0, s>>>(sums, nb, tot);
+  k_scan_apply<<<static_cast<int>(nb), kBlock, 0, s>>>(in, n, sums, tot, out);
+  LTRY(hipGetLastError(), "scan kernels");
+  LTRY(hipMemcpyAsync(total, tot, sizeof(int64_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return LJGPU_OK;
+}
+
+int dev_validate(hipStream_t s, int32_t n, const int32_t* nop, const int32_t* ptr,
+                 const int32_t* list, int64_t e, int kind, ListWork& w) {
+  if (n <= 0) return LJGPU_OK;
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  unsigned long long* bad = w.small.as<unsigned long long>();
+  LCHECK(fill<unsigned long long>(s, bad, 1, ~0ull));
+  k_validate<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, nop, ptr, list, e,
+                                                   kind == LJGPU_LIST_HALF ? 1 : 0, bad);
+  LTRY(hipGetLastError(), "validate kernel");
+  unsigned long long h = 0;
+  LTRY(hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (h == ~0ull) return LJGPU_OK;
+  const int64_t row = static_cast<int64_t>(h / 8);
+  static const char* what[] = {"", "number_of_partners out of range", "row leaves the list",
+                               "partner index out of range", "row lists itself",
+                               "half list holds j < i"};
+  return set_error(LJGPU_ERR_BAD_LIST,
+                   "row " + std::to_string(row) + ": " + what[std::min<unsigned long long>(h % 8, 5)]);
+}
+
+int dev_copy_row_ordered(hipStream_t s, int32_t n, const int32_t* nop, const int32_t* ptr,
+                         const int32_t* list, int64_t e, ListWork& w, DevCSR& out) {
+  (void)e;
+  out.n = n;
+  LCHECK(dev_reserve(out.nop, sizeof(int32_t) * std::max(n, 1), s));
+  LCHECK(dev_reserve(out.ptr, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  if (n > 0)
+    LTRY(hipMemcpyAsync(out.nop.p, nop, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s),
+         "hipMemcpyAsync");
+  int64_t total = 0;
+  LCHECK(dev_scan(s, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), n, w, &total));
+  if (total > INT32_MAX) return set_error(LJGPU_ERR_CAPACITY, "list longer than 2^31-1 entries");
+  out.e = total;
+  LCHECK(dev_reserve(out.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  if (n > 0) {
+    k_copy_rows<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(n, nop, ptr, list, out.ptr.as<int32_t>(),
+                                                           out.list.as<int32_t>());
+    LTRY(hipGetLastError(), "copy rows kernel");
+  }
+  return LJGPU_OK;
+}
+
+int dev_expand_half(hipStream_t s, int32_t n, const int32_t* hnop, const int32_t* hptr,
+                    const int32_t* hlist, ListWork& w, DevCSR& full) {
+  full.n = n;
+  LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  LCHECK(dev_reserve(w.cell_slot, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  LCHECK(dev_reserve(full.nop, sizeof(int32_t) * std::max(n, 1), s));
+  LCHECK(dev_reserve(full.ptr, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  int32_t* lower = w.counts.as<int32_t>();
+  int32_t* fillc = w.cell_slot.as<int32_t>();
+  LCHECK(fill<int32_t>(s, lower, n, 0));
+  LCHECK(fill<int32_t>(s, fillc, n, 0));
+  if (n > 0) {
+    k_count_lower<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, hnop, hptr, hlist, lower);
+    k_add<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, lower, hnop, full.nop.as<int32_t>());
+    LTRY(hipGetLastError(), "expand kernels");
+  }
+  int64_t total = 0;
+  LCHECK(dev_scan(s, full.nop.as<int32_t>(), full.ptr.as<int32_t>(), n, w, &total));
+  if (total > INT32_MAX) return set_error(LJGPU_ERR_CAPACITY, "full list longer than 2^31-1 entries");
+  full.e = total;
+  LCHECK(dev_reserve(full.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  LCHECK(dev_reserve(w.tmp, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  if (n > 0) {
+    k_fill_expanded<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+        n, hnop, hptr, hlist, lower, full.ptr.as<int32_t>(), fillc, w.tmp.as<int32_t>(),
+        full.list.as<int32_t>());
+    LTRY(hipGetLastError(), "expand fill kernel");
+    // Lower partners ascending (the order in which the reference's rows
+    // r < i list i).
+    LCHECK(row_sort(s, n, lower, full.ptr.as<int32_t>(), w.tmp.as<int32_t>(),
+                    full.list.as<int32_t>()));
+  }
+  return LJGPU_OK;
+}
+
+int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double search_length,
+                 int kind, ListWork& w, DevCSR& out) {
+  const double sl2 = search_length * search_length;
+  out.n = n;
+  LCHECK(dev_reserve(out.nop, sizeof(int32_t) * std::max(n, 1), s));
+  LCHECK(dev_reserve(out.ptr, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  if (n == 0) {
+    out.e = 0;
+    LCHECK(dev_reserve(out.list, sizeof(int32_t), s));
+    return fill<int32_t>(s, out.ptr.as<int32_t>(), 1, 0);
+  }
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  if (box <= 0.0) {
+    // Bounding box of the open system, reduced on the device.
+    std::vector<double> ext(6);
+    DevMem& m = w.scan_sums;
+    LCHECK(dev_reserve(m, sizeof(double) * 6, s));
+    const double* comp[3] = {q.x, q.y, q.z};
+    for (int c = 0; c < 3; c++)
+      k_extrema<<<1, kBlock, 0, s>>>(n, comp[c], q.stride, m.as<double>() + 2 * c);
+    LTRY(hipGetLastError(), "bounding box kernel");
+    LTRY(hipMemcpyAsync(ext.data(), m.p, sizeof(double) * 6, hipMemcpyDeviceToHost, s),
+         "hipMemcpyAsync");
+    LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    for (int c = 0; c < 3; c++) {
+      lo[c] = ext[2 * c];
+      hi[c] = ext[2 * c + 1];
+      if (!std::isfinite(lo[c]) || !std::isfinite(hi[c]))
+        return set_error(LJGPU_ERR_INVALID_ARGUMENT, "positions are not finite");
+    }
+  }
+  CellGrid cg;
+  make_cell_grid(n, lo, hi, box, search_length, &cg);
+  GridArgs g{cg.nc[0], cg.nc[1], cg.nc[2], cg.origin[0], cg.origin[1], cg.origin[2],
+             cg.width[0], cg.width[1], cg.width[2], box};
+  const int64_t ncell = cg.ncell;
+  LCHECK(dev_reserve(w.cell_count, sizeof(int32_t) * (ncell + 1), s));
+  LCHECK(dev_reserve(w.cell_start, sizeof(int32_t) * (ncell + 1), s));
+  LCHECK(dev_reserve(w.cell_slot, sizeof(int32_t) * n, s));
+  LCHECK(dev_reserve(w.cell_of, sizeof(int32_t) * n, s));
+  LCHECK(dev_reserve(w.cell_sorted, sizeof(int32_t) * n, s));
+  LCHECK(dev_reserve(w.cpos, sizeof(double4) * n, s));
+  int32_t* count = w.cell_count.as<int32_t>();
+  int32_t* cstart = w.cell_start.as<int32_t>();
+  LCHECK(fill<int32_t>(s, count, ncell, 0));
+  k_cell_assign<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, q, g, count, w.cell_slot.as<int32_t>(),
+                                                       w.cell_of.as<int32_t>());
+  LTRY(hipGetLastError(), "cell assign kernel");
+  int64_t total = 0;
+  LCHECK(dev_scan(s, count, cstart, ncell, w, &total));
+  k_cell_scatter<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, q, w.cell_of.as<int32_t>(),
+                                                        w.cell_slot.as<int32_t>(), cstart,
+                                                        w.cpos.as<double4>(),
+                                                        w.cell_sorted.as<int32_t>());
+  LTRY(hipGetLastError(), "cell scatter kernel");
+  const int stencil = cg.brute ? 1 : 3;
+  const bool half = kind == LJGPU_LIST_HALF;
+  const int grid = grid_for(n, kBlock, INT32_MAX);
+  auto pairs = [&](auto fill_tag, int32_t* nop, const int32_t* ptr, int32_t* outl) {
+    constexpr bool F = decltype(fill_tag)::value;
+    if (half)
+      k_pairs<F, true><<<grid, kBlock, 0, s>>>(n, w.cpos.as<double4>(), w.cell_sorted.as<int32_t>(),
+                                               cstart, g, stencil, sl2, nop, ptr, outl);
+    else
+      k_pairs<F, false><<<grid, kBlock, 0, s>>>(n, w.cpos.as<double4>(), w.cell_sorted.as<int32_t>(),
+                                                cstart, g, stencil, sl2, nop, ptr, outl);
+  };
+  pairs(std::false_type{}, out.nop.as<int32_t>(), nullptr, nullptr);
+  LTRY(hipGetLastError(), "pair count kernel");
+  LCHECK(dev_scan(s, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), n, w, &total));
+  if (total > INT32_MAX) return set_error(LJGPU_ERR_CAPACITY, "list longer than 2^31-1 entries");
+  out.e = total;
+  LCHECK(dev_reserve(out.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  LCHECK(dev_reserve(w.tmp, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  pairs(std::true_type{}, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>());
+  LTRY(hipGetLastError(), "pair fill kernel");
+  return row_sort(s, n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(),
+                  out.list.as<int32_t>());
+}
+
+int dev_minmax(hipStream_t s, const int32_t* a, int32_t n, ListWork& w, int32_t* mn,
+               int32_t* mx) {
+  *mn = 0;
+  *mx = 0;
+  if (n <= 0) return LJGPU_OK;
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  int32_t* mm = w.small.as<int32_t>();
+  LCHECK(fill<int32_t>(s, mm, 1, INT32_MAX));
+  LCHECK(fill<int32_t>(s, mm + 1, 1, INT32_MIN));
+  k_minmax<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(n, a, mm);
+  LTRY(hipGetLastError(), "minmax kernel");
+  int32_t h[2];
+  LTRY(hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  *mn = h[0];
+  *mx = h[1];
+  return LJGPU_OK;
+}
+
+int dev_half_row_stats(hipStream_t s, const DevCSR& full, ListWork& w, int32_t* min_np,
+                       int32_t* max_np) {
+  if (full.n <= 0) return dev_minmax(s, nullptr, 0, w, min_np, max_np);
+  LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(full.n) + 1), s));
+  k_upper_counts<<<grid_for(full.n, kBlock), kBlock, 0, s>>>(
+      full.n, full.nop.as<int32_t>(), full.ptr.as<int32_t>(), full.list.as<int32_t>(),
+      w.counts.as<int32_t>());
+  LTRY(hipGetLastError(), "upper count kernel");
+  return dev_minmax(s, w.counts.as<int32_t>(), full.n, w, min_np, max_np);
+}
+
+int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell) {
+  width = std::max(width, 1);
+  const int64_t cells = static_cast<int64_t>(c.n) * width;
+  LCHECK(dev_reserve(ell, sizeof(int32_t) * std::max<int64_t>(cells, 1), s));
+  if (c.n > 0) {
+    k_ell<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, width, c.nop.as<int32_t>(),
+                                                   c.ptr.as<int32_t>(), c.list.as<int32_t>(),
+                                                   ell.as<int32_t>());
+    LTRY(hipGetLastError(), "ELL kernel");
+  }
+  return LJGPU_OK;
+}
+
+int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, ListWork& w, DevTiles& out) {
+  out.tile_particles = 0;
+  out.blocks = 0;
+  if (T <= 0 || c.n <= 0) return LJGPU_OK;
+  const int32_t nb = (c.n + T - 1) / T;
+  LCHECK(dev_reserve(w.tile_ns, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
+  LCHECK(dev_reserve(w.tile_soff, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  int32_t* stats = w.small.as<int32_t>() + 4;
+  LCHECK(fill<int32_t>(s, stats, 3, 0));
+  const int grid = std::min(nb, 65535);
+  k_tile_count<<<grid, kBlock, 0, s>>>(c.n, T, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+                                       c.list.as<int32_t>(), w.tile_ns.as<int32_t>(), stats);
+  LTRY(hipGetLastError(), "tile count kernel");
+  int32_t h[3];
+  LTRY(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (h[0] != 0) return LJGPU_OK;  // does not fit: caller retries smaller or runs untiled
+  int64_t staged = 0;
+  LCHECK(dev_scan(s, w.tile_ns.as<int32_t>(), w.tile_soff.as<int32_t>(), nb, w, &staged));
+  if (staged > INT32_MAX) return LJGPU_OK;
+  LCHECK(dev_reserve(out.desc, sizeof(int32_t) * 8 * static_cast<size_t>(nb), s));
+  LCHECK(dev_reserve(out.stage, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
+  // The force kernel copies entry offsets in whole 16-byte chunks: pad.
+  const int64_t nlocal = c.e + kTileLocalPad;
+  LCHECK(dev_reserve(out.local, sizeof(uint16_t) * nlocal, s));
+  LCHECK(fill<uint16_t>(s, out.local.as<uint16_t>() + c.e, kTileLocalPad, 0));
+  k_tile_fill<<<grid, kBlock, 0, s>>>(c.n, T, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+                                      c.list.as<int32_t>(), w.tile_soff.as<int32_t>(),
+                                      out.desc.as<int32_t>(), out.stage.as<int32_t>(),
+                                      out.local.as<uint16_t>());
+  LTRY(hipGetLastError(), "tile fill kernel");
+  out.tile_particles = T;
+  out.blocks = nb;
+  out.staged = staged;
+  out.max_stage = h[1];
+  out.lds_bytes = h[2];
+  return LJGPU_OK;
+}
+
+}  // namespace ljgpu

@@ -7,6 +7,12 @@
 
 #include "../../include/ljgpu.h"
 
+#if defined(__HIPCC__)
+#define LJ_HD __host__ __device__
+#else
+#define LJ_HD
+#endif
+
 namespace ljgpu {
 
 // Thread-local message of the last failing call; returns `code`.
@@ -73,9 +79,24 @@ struct HostTiles {
 constexpr int32_t kTileLdsHeader = 16;  // kernel's own word(s) ahead of the tile data
 constexpr int32_t kTileLdsBudget = 65536 - kTileLdsHeader;
 constexpr int32_t kTileLocalPad = 8;
-inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
+LJ_HD inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
   return ((ns * 24 + 15) / 16) * 16 + ((ne + 14 + 7) / 8) * 16;
 }
+
+// Cell grid of the Verlet-list builders: cells at least the search length
+// wide (so a pair within the search length lies in the same or adjacent
+// cells), at most about 8 per particle. A periodic box with fewer than three
+// cells along a side falls back to one cell holding every particle (brute).
+struct CellGrid {
+  int nc[3] = {1, 1, 1};
+  double origin[3] = {0, 0, 0};
+  double extent[3] = {0, 0, 0};
+  double width[3] = {1, 1, 1};
+  bool brute = false;
+  int64_t ncell = 1;
+};
+void make_cell_grid(int32_t n, const double lo[3], const double hi[3], double box,
+                    double search_length, CellGrid* g);
 
 // True if rows are stored in order: pointer[i+1] == pointer[i] + nop[i].
 bool csr_is_row_ordered(const HostCSR& c);

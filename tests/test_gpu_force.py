@@ -39,7 +39,10 @@ def run_aos(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, block=256,
     tiled = tile > 0 and lanes > 1 and not (csr.kind == G.LIST_HALF and half_mode == G.HALF_ATOMIC)
     assert (info.tile_particles > 0) == tiled
     if tiled:
-        assert info.tile_particles == min(tile, 1024 // lanes)
+        # The requested size, capped at 1024 / lanes, halved while a block
+        # overflows the LDS budget.
+        want = min(tile, 1024 // lanes)
+        assert info.tile_particles in [want >> s for s in range(8)]
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)
     if mixed is None:

@@ -1,0 +1,187 @@
+"""Neighbour lists built and derived on the device.
+
+ljgpu_build_list builds the Verlet list from device positions with a cell
+list; the CSR it installs must be bit-identical to the host builder's
+(ljgpu_makepair_host), which the CPU suite pins to the oracle's O(N^2)
+makepair (tests/test_host_lists.py) and to the compiled reference's CSR
+(tests/test_oracle_golden.py). Layouts derived on the device from a CSR (the
+full list of a half list, row ordering, LDS tiles) must equal their host
+restatements (expand_half_to_full via makepair FULL, ljgpu_make_tiles).
+"""
+import numpy as np
+import pytest
+
+import lj_gpu_amd as G
+from helpers import FP64_TOL, PBC_DT, normwise, oracle_momenta, periodic
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if G.device_count() < 1:
+        pytest.skip("no HIP device")
+    c = G.Context(0)
+    yield c
+    c.close()
+
+
+def _same(a: G.CSR, b: G.CSR) -> None:
+    assert a.n == b.n
+    np.testing.assert_array_equal(a.number_of_partners, b.number_of_partners)
+    np.testing.assert_array_equal(a.pointer, b.pointer)
+    np.testing.assert_array_equal(a.sorted_list, b.sorted_list)
+
+
+def _device_build(ctx, q, box, kind, half_mode=G.HALF_AUTO, soa=False):
+    ctx.set_half_mode(half_mode)
+    if soa:
+        dq = [ctx.array(np.ascontiguousarray(q[:, c])) for c in range(3)]
+        ctx.build_list_soa(dq, q.shape[0], box, kind)
+    else:
+        dq = ctx.array(q)
+        ctx.build_list(dq, q.shape[0], box, kind)
+    return ctx.get_list(), ctx.list_info()
+
+
+@pytest.mark.parametrize("k,rho", [(5, 1.0), (12, 1.0), (31, 1.0), (31, 0.5)])
+@pytest.mark.parametrize("kind", [G.LIST_HALF, G.LIST_FULL])
+def test_device_build_matches_host_periodic(ctx, k, rho, kind):
+    q, box, half, full = periodic(k, rho)
+    host = half if kind == G.LIST_HALF else full
+    print(f"build k={k} rho={rho} kind={kind}", flush=True)
+    # Half lists: the atomic mode evaluates the half list itself, the
+    # default expands it; both are checked.
+    got, info = _device_build(ctx, q, box, kind, G.HALF_ATOMIC)
+    _same(got, host)
+    assert info.n_entries == host.n_entries
+    assert info.max_partners == host.number_of_partners.max()
+    assert info.min_partners == host.number_of_partners.min()
+    if kind == G.LIST_HALF:
+        got, info = _device_build(ctx, q, box, kind, G.HALF_EXPAND)
+        _same(got, full)
+        assert info.n_entries == half.n_entries
+        assert info.max_partners == half.number_of_partners.max()
+        assert info.min_partners == half.number_of_partners.min()
+
+
+@pytest.mark.parametrize("density", [0.5, 1.0])
+def test_device_build_matches_host_reference_box(ctx, density):
+    """The reference's own open-box problem (init() with L = 50)."""
+    q = G.lattice_reference(density, 50.0)
+    host = G.makepair(q, 0.0, G.LIST_HALF)
+    got, _ = _device_build(ctx, q, 0.0, G.LIST_HALF, G.HALF_ATOMIC)
+    _same(got, host)
+
+
+def test_device_build_random_clouds(ctx):
+    rng = np.random.default_rng(5)
+    for n, side, box in [(3000, 20.0, 0.0), (3000, 20.0, 20.0), (500, 7.0, 7.0), (50, 100.0, 0.0),
+                         (1, 3.0, 0.0), (2, 10.0, 10.0)]:
+        q = np.zeros((n, 4))
+        q[:, :3] = rng.uniform(0.0, side, size=(n, 3))
+        for kind in (G.LIST_HALF, G.LIST_FULL):
+            host = G.makepair(q, box, kind)
+            got, _ = _device_build(ctx, q, box, kind, G.HALF_ATOMIC)
+            _same(got, host)
+
+
+def test_device_build_soa_matches_aos(ctx):
+    q, box, half, full = periodic(12, 1.0)
+    got, _ = _device_build(ctx, q, box, G.LIST_FULL, soa=True)
+    _same(got, full)
+
+
+def test_device_build_rejects_bad_arguments(ctx):
+    q, box, _, _ = periodic(5, 1.0)
+    dq = ctx.array(q)
+    with pytest.raises(G.LJGPUError):
+        ctx.build_list(dq, q.shape[0], 5.0, G.LIST_FULL)  # box < 2 * search length
+    with pytest.raises(G.LJGPUError):
+        ctx.build_list(dq, q.shape[0], box, 7)
+
+
+def test_forces_from_device_built_list(ctx):
+    q, box, half, full = periodic(12, 1.0)
+    ref = oracle_momenta(12, 1.0)
+    ctx.set_kernel_config(8, 256, 4)
+    ctx.set_tiling(32)
+    ctx.set_half_mode(G.HALF_AUTO)
+    dq = ctx.array(q)
+    ctx.build_list(dq, q.shape[0], box, G.LIST_HALF)
+    dp = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, 100)
+    ctx.sync()
+    assert normwise(dp.download()[:, :3], ref) < FP64_TOL
+
+
+def test_empty_system(ctx):
+    q = np.zeros((0, 4))
+    dq = ctx.zeros((1, 4))
+    ctx.build_list(dq, 0, 0.0, G.LIST_FULL)
+    assert ctx.list_info().n_entries == 0
+
+
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
+@pytest.mark.parametrize("lanes,tile", [(8, 32), (16, 16), (4, 64)])
+def test_device_tiles_match_host_tiles(ctx, k, rho, lanes, tile):
+    q, box, half, full = periodic(k, rho)
+    ctx.set_kernel_config(lanes, 256, 2)
+    ctx.set_tiling(tile)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    t = ctx.get_tiles()
+    assert t is not None
+    info = ctx.list_info()
+    h = G.make_tiles(full, info.tile_particles)
+    # Identical whenever the host tiler did not have to shrink a block.
+    if np.all(h.desc[:-1, 1] == info.tile_particles):
+        np.testing.assert_array_equal(t.desc, h.desc)
+        np.testing.assert_array_equal(t.stage, h.stage)
+        np.testing.assert_array_equal(t.local, h.local)
+    assert t.max_stage == info.tile_max_stage
+
+
+def test_device_expansion_of_shuffled_half_list(ctx):
+    """Half list with shuffled rows, installed from device memory: the
+    expansion keeps lower partners ascending and the own row as stored."""
+    q, box, half, full = periodic(6, 1.0)
+    sh = G.random_shfl(half, 10)
+    n = sh.n
+    nop, ptr, lst = (ctx.array(a) for a in (sh.number_of_partners, sh.pointer, sh.sorted_list))
+    ctx.set_half_mode(G.HALF_EXPAND)
+    ctx.set_kernel_config(8, 256, 4)
+    ctx.set_list_device(n, nop, ptr, lst, sh.n_entries, G.LIST_HALF)
+    got = ctx.get_list()
+    for i in range(n):
+        row = got.sorted_list[got.pointer[i]: got.pointer[i] + got.number_of_partners[i]]
+        lower = full.sorted_list[full.pointer[i]: full.pointer[i] + full.number_of_partners[i]]
+        lower = lower[lower < i]
+        own = sh.sorted_list[sh.pointer[i]: sh.pointer[i] + sh.number_of_partners[i]]
+        np.testing.assert_array_equal(row, np.concatenate([lower, own]))
+
+
+def test_device_validation_of_device_lists(ctx):
+    q, box, half, full = periodic(5, 1.0)
+    for mutate, kind in [(lambda l, p, c: l.__setitem__(3, -1), G.LIST_FULL),
+                         (lambda l, p, c: l.__setitem__(p[5], 5), G.LIST_FULL),
+                         (lambda l, p, c: p.__setitem__(0, 10 ** 6), G.LIST_FULL),
+                         (lambda l, p, c: c.__setitem__(2, -3), G.LIST_FULL),
+                         (lambda l, p, c: l.__setitem__(p[9], 1), G.LIST_HALF)]:
+        src = full if kind == G.LIST_FULL else half
+        lst, ptr, nop = src.sorted_list.copy(), src.pointer.copy(), src.number_of_partners.copy()
+        mutate(lst, ptr, nop)
+        d = [ctx.array(a) for a in (nop, ptr, lst)]
+        with pytest.raises(G.LJGPUError) as e:
+            ctx.set_list_device(src.n, d[0], d[1], d[2], src.n_entries, kind)
+        assert e.value.code == G.ERR_BAD_LIST
+
+
+@pytest.mark.slow
+def test_device_build_large_system(ctx):
+    """FCC 4*48^3 = 442,368 particles at density 0.8 (the dynamic-system
+    density), full list."""
+    q, box = G.lattice_periodic(48, 0.8, 2)
+    host = G.makepair(q, box, G.LIST_FULL)
+    got, _ = _device_build(ctx, q, box, G.LIST_FULL)
+    _same(got, host)
