@@ -229,6 +229,78 @@ int ljgpu_force_kernel_csr_f64(ljgpu_ctx* ctx, const double* q, double* p,
                                const int32_t* sorted_list, const int32_t* number_of_partners,
                                const int32_t* pointer, int64_t n_entries, int kind);
 
+/* ---- molecular dynamics around the force path ----------------------------- */
+/* No reference counterpart: the reference has no integrator and no energy,
+ * it repeats force evaluations on fixed positions (measure()'s loop,
+ * cuda/force_cuda.cu:331-335). These entry points wrap that same evaluation
+ * (p += F(q) * dt, unit masses, so p is the velocity) in a velocity-Verlet
+ * loop with a periodic list rebuild on the device. */
+#define LJGPU_PREC_F64 0
+#define LJGPU_PREC_MIXED 1
+
+typedef struct ljgpu_md_config {
+  double dt;              /* time step                                          */
+  double cutoff;          /* force cutoff: pairs with r2 > cutoff^2 are skipped  */
+  double skin;            /* Verlet skin: lists hold pairs within cutoff + skin  */
+  int32_t rebuild_every;  /* rebuild the list every this many steps; 0: whenever
+                             a particle has moved more than skin/2 since the
+                             last build (checked every step)                   */
+  int32_t precision;      /* LJGPU_PREC_F64 or LJGPU_PREC_MIXED (forces; the
+                             integration is FP64 either way)                    */
+  int32_t accum;          /* LJGPU_ACCUM_* for mixed precision                   */
+  int32_t sample_every;   /* energy sample interval (0: first and last step)     */
+} ljgpu_md_config;
+
+typedef struct ljgpu_md_stats {
+  int32_t steps;
+  int32_t rebuilds;        /* list builds, including the initial one           */
+  int32_t samples;         /* energy samples taken                              */
+  int32_t stale_lists;     /* rebuilds at which some particle had moved more
+                              than skin/2 since the previous build (the list may
+                              have missed pairs: rebuild more often)           */
+  double potential0, kinetic0;  /* energies at the first sample (step 0); the
+                                   potential is shifted: each pair inside the
+                                   cutoff counts U(r) - U(cutoff), so that
+                                   U + K is the energy truncated forces
+                                   conserve                                   */
+  double potential1, kinetic1;  /* energies at the last sample (final step)    */
+  double max_abs_drift;    /* max |E(t) - E(0)| over the samples, E = U + K    */
+  double max_displacement; /* largest displacement found at a list check        */
+  int64_t pair_entries;    /* list entries evaluated by force kicks            */
+  float ms_total;          /* GPU time of the run                              */
+  float ms_rebuild;        /* part of it spent building lists                  */
+} ljgpu_md_stats;
+
+/* Run `steps` velocity-Verlet steps in place on device AoS q and p (N x 4,
+ * 16-byte aligned): positions wrapped into [0, box) for box > 0, the full
+ * list rebuilt on the device (ljgpu_build_list, search length cutoff + skin)
+ * at the start and every rebuild_every steps, potential and kinetic energy
+ * sampled at step 0, every sample_every steps and at the end. `samples`
+ * (optional, host) receives up to sample_capacity rows (step, U, K, U + K)
+ * with U the shifted potential (see ljgpu_md_stats).
+ * The context's list is replaced by the run's list. */
+int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t particle_number, double box,
+                 const ljgpu_md_config* config, int32_t steps, ljgpu_md_stats* stats,
+                 double* samples, int32_t sample_capacity);
+
+/* Potential energy sum of 4 (r^-12 - r^-6) over the installed list's pairs
+ * with r2 <= cl2 (the force kernels' cutoff rule, no shift), kinetic energy
+ * sum of |p|^2 / 2, device AoS q and p, and (optional) the number of pairs
+ * inside the cutoff. Synchronous. */
+int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
+                 double* potential, double* kinetic, int64_t* pairs_in_cutoff);
+
+/* Maxwell-Boltzmann momenta at `temperature` (unit masses): each component
+ * N(0, temperature) from std::mt19937(seed) in particle order x, y, z, then
+ * the mean momentum removed. Host AoS p[N][4] (pad set to 0). */
+int ljgpu_maxwell_boltzmann(int32_t particle_number, double temperature, uint32_t seed,
+                            double* p);
+
+/* Run repeated force evaluations (steps >= 4) and the MD driver's plain steps
+ * as replays of captured HIP graphs (1, the default) or as individual
+ * launches (0). */
+int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable);
+
 /* ---- host-side setup (CPU only, no device needed) ----------------------- */
 /* The reference's init(): FCC lattice in an open box of side box_length,
  * jitter U(0, 0.1) from std::mt19937(2), AoS double[N][4].

@@ -83,6 +83,29 @@ _i64 = C.c_int64
 _d = C.c_double
 
 
+PREC_F64 = 0
+PREC_MIXED = 1
+
+
+class MDConfig(C.Structure):
+    """ljgpu_md_config (include/ljgpu.h)."""
+    _fields_ = [("dt", C.c_double), ("cutoff", C.c_double), ("skin", C.c_double),
+                ("rebuild_every", C.c_int32), ("precision", C.c_int32), ("accum", C.c_int32),
+                ("sample_every", C.c_int32)]
+
+
+class MDStats(C.Structure):
+    """ljgpu_md_stats (include/ljgpu.h)."""
+    _fields_ = [("steps", C.c_int32), ("rebuilds", C.c_int32), ("samples", C.c_int32),
+                ("stale_lists", C.c_int32), ("potential0", C.c_double), ("kinetic0", C.c_double),
+                ("potential1", C.c_double), ("kinetic1", C.c_double),
+                ("max_abs_drift", C.c_double), ("max_displacement", C.c_double),
+                ("pair_entries", C.c_int64), ("ms_total", C.c_float), ("ms_rebuild", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib() -> C.CDLL:
     """Load libljgpu.so (built by lj_gpu_amd/build.py); raises if absent."""
     global _lib
@@ -128,6 +151,11 @@ def lib() -> C.CDLL:
         "ljgpu_build_list_soa": [_vp, _vp, _vp, _vp, _i32, _d, _d, C.c_int],
         "ljgpu_get_tiles": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i32),
                             C.POINTER(_i64)],
+        "ljgpu_md_run": [_vp, _vp, _vp, _i32, _d, C.POINTER(MDConfig), _i32, C.POINTER(MDStats),
+                         _vp, _i32],
+        "ljgpu_energy": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
+        "ljgpu_maxwell_boltzmann": [_i32, _d, C.c_uint32, _vp],
+        "ljgpu_set_graphs": [_vp, C.c_int],
         "ljgpu_pair_dat_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64, _i32, _i64],
         "ljgpu_pair_dat_read": [C.c_char_p, _i32, _i32, _i64, _vp, _vp, _vp, _i64,
                                 C.POINTER(_i64)],
@@ -250,6 +278,13 @@ def make_transposed(csr: CSR, width: int | None = None) -> np.ndarray:
     _check(lib().ljgpu_make_transposed(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
                                        _ptr(csr.sorted_list), width, _ptr(out)))
     return out[: csr.n * width]
+
+
+def maxwell_boltzmann(n: int, temperature: float, seed: int) -> np.ndarray:
+    """Seeded Maxwell-Boltzmann momenta (unit masses), (N, 4), zero mean."""
+    p = np.zeros((n, 4), np.float64)
+    _check(lib().ljgpu_maxwell_boltzmann(n, temperature, seed, _ptr(p)))
+    return p
 
 
 def write_pair_dat(path: str, csr: CSR, n_capacity: int = 0, pair_capacity: int = 0) -> None:
@@ -463,6 +498,36 @@ class Context:
                                      _ptr(local), local.size, C.byref(nb), C.byref(ns)))
         d = desc.reshape(-1, 8)
         return Tiles(d, stage[: ns.value], local[:e], int(d[:, 3].max()))
+
+    def set_graphs(self, enable: bool) -> None:
+        _check(lib().ljgpu_set_graphs(self.handle, 1 if enable else 0))
+
+    def energy(self, q: DeviceArray, p: DeviceArray, box: float = 0.0,
+               cl2: float = CL2) -> tuple[float, float]:
+        """(potential, kinetic) over the installed list (unshifted potential)."""
+        return self.energy_pairs(q, p, box, cl2)[:2]
+
+    def energy_pairs(self, q: DeviceArray, p: DeviceArray, box: float = 0.0,
+                     cl2: float = CL2) -> tuple[float, float, int]:
+        """(potential, kinetic, pairs inside the cutoff)."""
+        u, k, c = _d(0.0), _d(0.0), _i64(0)
+        _check(lib().ljgpu_energy(self.handle, q.ptr, p.ptr, box, cl2, C.byref(u), C.byref(k),
+                                  C.byref(c)))
+        return u.value, k.value, c.value
+
+    def md_run(self, q: DeviceArray, p: DeviceArray, box: float, steps: int, dt: float = 0.005,
+               cutoff: float = CUTOFF_LENGTH, skin: float = 0.3, rebuild_every: int = 20,
+               precision: int = PREC_F64, accum: int = ACCUM_F64, sample_every: int = 0,
+               max_samples: int = 0) -> tuple[MDStats, np.ndarray]:
+        """Velocity-Verlet run in place on device AoS q, p; returns the stats
+        and the energy samples (rows: step, U, K, U + K)."""
+        cfg = MDConfig(dt, cutoff, skin, rebuild_every, precision, accum, sample_every)
+        st = MDStats()
+        cap = max_samples if max_samples > 0 else (steps // sample_every + 2 if sample_every else 2)
+        buf = np.zeros((cap, 4), np.float64)
+        _check(lib().ljgpu_md_run(self.handle, q.ptr, p.ptr, q.shape[0], box, C.byref(cfg), steps,
+                                  C.byref(st), _ptr(buf), cap))
+        return st, buf[: min(st.samples, cap)].copy()
 
     def list_info(self) -> ListInfo:
         info = ListInfo()

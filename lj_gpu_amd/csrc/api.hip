@@ -4,71 +4,16 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
 
-#include "ljgpu_internal.h"
-#include "lists.h"
+#include "context.h"
 
 using namespace ljgpu;
 
-struct ljgpu_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  bool timer_running = false;
-
-  // List as given by the caller (or built by ljgpu_build_list).
-  int32_t n = 0;
-  int kind = -1;
-  int64_t n_entries = 0;
-  int32_t max_np = 0, min_np = 0;
-
-  // List as evaluated on the device: row-ordered CSR (the full list, or the
-  // half list itself for atomic mode), and the layouts derived from it.
-  int half_mode_req = LJGPU_HALF_AUTO;
-  int half_mode = 0;
-  bool eval_half = false;
-  DevCSR eval;
-  DevCSR staging;  // a caller's host list on its way to `eval`
-  DevMem ell;
-  int32_t ell_width = 0;
-  bool ell_ready = false;
-
-  // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
-  int32_t tile_particles = 32;
-  bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
-  int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
-  int cus = 0;                    // compute units of the device
-  int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
-  DevTiles tiles;
-
-  ListWork work;
-
-  int lanes = 8;
-  int unroll = 4;
-  int block = 256;
-};
-
-namespace {
-
-int hip_fail(hipError_t e, const char* what) {
-  return set_error(e == hipErrorOutOfMemory ? LJGPU_ERR_OUT_OF_MEMORY : LJGPU_ERR_HIP,
-                   std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(expr, what)                 \
-  do {                                      \
-    hipError_t e_ = (expr);                 \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
-#define TRY(call)         \
-  do {                    \
-    int rc_ = (call);     \
-    if (rc_) return rc_;  \
-  } while (0)
+namespace ljgpu {
 
 int use_device(ljgpu_ctx* ctx) {
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
@@ -244,50 +189,118 @@ int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps) {
   return LJGPU_OK;
 }
 
-int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, int accum) {
-  int rc = use_device(ctx);
-  if (rc) return rc;
-  if (ctx->n == 0 || steps == 0) return LJGPU_OK;
-  ListView lv;
-  lv.n = ctx->n;
-  lv.nop = ctx->eval.nop.as<int32_t>();
-  lv.ptr = ctx->eval.ptr.as<int32_t>();
-  lv.n_entries = ctx->eval.e;
-  LaunchConfig lc;
-  lc.lanes = ctx->lanes;
-  lc.unroll = ctx->unroll;
-  lc.block = ctx->block;
-  lc.tile_offsets_lds = ctx->tile_offsets_lds;
-  lc.tile_resident = ctx->tile_resident;
-  lc.cus = ctx->cus;
-  lc.half_atomic = ctx->eval_half;
-  lc.clamp = false;
+// Kernel arguments of one force evaluation over the installed list.
+int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc) {
   TRY(prepare_layouts(ctx));
+  *lv = ListView();
+  lv->n = ctx->n;
+  lv->nop = ctx->eval.nop.as<int32_t>();
+  lv->ptr = ctx->eval.ptr.as<int32_t>();
+  lv->n_entries = ctx->eval.e;
+  *lc = LaunchConfig();
+  lc->lanes = ctx->lanes;
+  lc->unroll = ctx->unroll;
+  lc->block = ctx->block;
+  lc->tile_offsets_lds = ctx->tile_offsets_lds;
+  lc->tile_resident = ctx->tile_resident;
+  lc->cus = ctx->cus;
+  lc->half_atomic = ctx->eval_half;
+  lc->clamp = false;
   if (ctx->lanes == 1) {
-    lv.list = ctx->ell.as<int32_t>();
-    lv.ell_width = ctx->ell_width;
+    lv->list = ctx->ell.as<int32_t>();
+    lv->ell_width = ctx->ell_width;
   } else {
-    lv.list = ctx->eval.list.as<int32_t>();
+    lv->list = ctx->eval.list.as<int32_t>();
     if (ctx->tiles.blocks > 0 && !ctx->eval_half) {
-      lv.tile_blocks = ctx->tiles.blocks;
-      lv.tile_particles = ctx->tiles.tile_particles;
-      lv.tile_lds_bytes = ctx->tiles.lds_bytes;
-      lv.tile_max_stage = ctx->tiles.max_stage;
-      lv.tile_desc = ctx->tiles.desc.as<int32_t>();
-      lv.tile_stage = ctx->tiles.stage.as<int32_t>();
-      lv.tile_local = ctx->tiles.local.as<uint16_t>();
+      lv->tile_blocks = ctx->tiles.blocks;
+      lv->tile_particles = ctx->tiles.tile_particles;
+      lv->tile_lds_bytes = ctx->tiles.lds_bytes;
+      lv->tile_max_stage = ctx->tiles.max_stage;
+      lv->tile_desc = ctx->tiles.desc.as<int32_t>();
+      lv->tile_stage = ctx->tiles.stage.as<int32_t>();
+      lv->tile_local = ctx->tiles.local.as<uint16_t>();
     }
-  }
-  for (int32_t s = 0; s < steps; s++) {
-    rc = !mixed ? launch_force_f64(ctx->stream, lv, fa, lc)
-         : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(ctx->stream, lv, fa, lc)
-                                          : launch_force_mixed_f64acc(ctx->stream, lv, fa, lc);
-    if (rc) return rc;
   }
   return LJGPU_OK;
 }
 
-}  // namespace
+int launch_one(ljgpu_ctx* ctx, const ListView& lv, const LaunchConfig& lc, const ForceArgs& fa,
+               bool mixed, int accum) {
+  return !mixed ? launch_force_f64(ctx->stream, lv, fa, lc)
+         : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(ctx->stream, lv, fa, lc)
+                                          : launch_force_mixed_f64acc(ctx->stream, lv, fa, lc);
+}
+
+int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum) {
+  if (ctx->n == 0) return LJGPU_OK;
+  ListView lv;
+  LaunchConfig lc;
+  TRY(force_view(ctx, &lv, &lc));
+  return launch_one(ctx, lv, lc, fa, mixed, accum);
+}
+
+void drop_graph(ljgpu_ctx* ctx) {
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  ctx->graph_exec = nullptr;
+  ctx->graph_steps = 0;
+}
+
+int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out) {
+  *out = nullptr;
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+          "hipStreamBeginCapture");
+  const int rc = body();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  const hipError_t e2 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) return hip_fail(e2, "hipGraphInstantiate");
+  return LJGPU_OK;
+}
+
+int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, int accum) {
+  TRY(use_device(ctx));
+  if (ctx->n == 0 || steps == 0) return LJGPU_OK;
+  ListView lv;
+  LaunchConfig lc;
+  TRY(force_view(ctx, &lv, &lc));
+  int32_t done = 0;
+  if (ctx->use_graphs && steps >= kGraphMinSteps) {
+    // Repeated evaluations run as replays of one captured graph of up to
+    // kGraphChunk launches, re-captured only when an argument changes.
+    const int32_t chunk = std::min(steps, kGraphChunk);
+    StepKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.fa = fa;
+    key.lv = lv;
+    key.lc = lc;
+    key.mixed = mixed;
+    key.accum = accum;
+    key.steps = chunk;
+    if (!ctx->graph_exec || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
+      drop_graph(ctx);
+      hipGraphExec_t exec = nullptr;
+      TRY(capture(ctx, [&]() {
+        for (int32_t s = 0; s < chunk; s++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
+        return LJGPU_OK;
+      }, &exec));
+      ctx->graph_exec = exec;
+      ctx->graph_key = key;
+      ctx->graph_steps = chunk;
+    }
+    for (; done + chunk <= steps; done += chunk)
+      HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream), "hipGraphLaunch");
+  }
+  for (; done < steps; done++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
+  return LJGPU_OK;
+}
+
+}  // namespace ljgpu
 
 extern "C" {
 
@@ -332,6 +345,7 @@ int ljgpu_destroy(ljgpu_ctx* ctx) {
   if (!ctx) return LJGPU_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
   release_all(ctx);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

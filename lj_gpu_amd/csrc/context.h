@@ -1,0 +1,119 @@
+// context.h — the library context (struct ljgpu_ctx) and the internal
+// operations api.hip and md.hip share. HIP-only header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+
+#include "ljgpu_internal.h"
+#include "lists.h"
+
+namespace ljgpu {
+
+// Graph replay of repeated force evaluations: calls with at least
+// kGraphMinSteps steps run as replays of one captured graph of up to
+// kGraphChunk launches.
+constexpr int32_t kGraphMinSteps = 4;
+constexpr int32_t kGraphChunk = 100;
+
+// Every argument of a captured force loop; a graph is replayed only for an
+// identical key (compared bytewise, so it is zero-filled before use).
+struct StepKey {
+  ForceArgs fa;
+  ListView lv;
+  LaunchConfig lc;
+  int32_t mixed;
+  int32_t accum;
+  int32_t steps;
+};
+
+}  // namespace ljgpu
+
+struct ljgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool timer_running = false;
+
+  // List as given by the caller (or built by ljgpu_build_list).
+  int32_t n = 0;
+  int kind = -1;
+  int64_t n_entries = 0;
+  int32_t max_np = 0, min_np = 0;
+
+  // List as evaluated on the device: row-ordered CSR (the full list, or the
+  // half list itself for atomic mode), and the layouts derived from it.
+  int half_mode_req = LJGPU_HALF_AUTO;
+  int half_mode = 0;
+  bool eval_half = false;
+  ljgpu::DevCSR eval;
+  ljgpu::DevCSR staging;  // a caller's host list on its way to `eval`
+  ljgpu::DevMem ell;
+  int32_t ell_width = 0;
+  bool ell_ready = false;
+
+  // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
+  int32_t tile_particles = 32;
+  bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
+  int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
+  int cus = 0;                    // compute units of the device
+  int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
+  ljgpu::DevTiles tiles;
+
+  ljgpu::ListWork work;
+
+  // Captured force loop (see StepKey).
+  bool use_graphs = true;
+  hipGraphExec_t graph_exec = nullptr;
+  ljgpu::StepKey graph_key{};
+  int32_t graph_steps = 0;
+
+  // Molecular-dynamics state: positions at the last list build, reduction
+  // scratch.
+  ljgpu::DevMem md_qref;
+  ljgpu::DevMem md_red;
+
+  int lanes = 8;
+  int unroll = 4;
+  int block = 256;
+};
+
+namespace ljgpu {
+
+inline int hip_fail(hipError_t e, const char* what) {
+  return set_error(e == hipErrorOutOfMemory ? LJGPU_ERR_OUT_OF_MEMORY : LJGPU_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                              \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return ::ljgpu::hip_fail(e_, what); \
+  } while (0)
+
+#define TRY(call)         \
+  do {                    \
+    int rc_ = (call);     \
+    if (rc_) return rc_;  \
+  } while (0)
+
+int use_device(ljgpu_ctx* ctx);
+int prepare_layouts(ljgpu_ctx* ctx);
+// Build the list from device positions and install it (ljgpu_build_list).
+int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind);
+// Enqueue one force evaluation p += F(q) * dt over the installed list.
+int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum);
+// `steps` force evaluations (graph replays where possible).
+int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, int accum);
+void drop_graph(ljgpu_ctx* ctx);
+// Kernel arguments of one force evaluation over the installed list (derives
+// missing layouts first).
+int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc);
+// Record the launches `body` enqueues on the context stream into an
+// executable graph.
+int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out);
+
+}  // namespace ljgpu

@@ -412,6 +412,25 @@ int ljgpu_makepair_host(int32_t n, const double* q, double box, double search_le
   return LJGPU_OK;
 }
 
+int ljgpu_maxwell_boltzmann(int32_t n, double temperature, uint32_t seed, double* p) {
+  if (n < 0 || (n > 0 && !p) || !(temperature >= 0.0))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "maxwell_boltzmann: bad argument");
+  std::mt19937 mt(seed);
+  std::normal_distribution<double> nd(0.0, std::sqrt(temperature));
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int32_t i = 0; i < n; i++) {
+    for (int c = 0; c < 3; c++) {
+      p[4 * (int64_t)i + c] = nd(mt);
+      mean[c] += p[4 * (int64_t)i + c];
+    }
+    p[4 * (int64_t)i + 3] = 0.0;
+  }
+  for (int c = 0; c < 3; c++) mean[c] = n > 0 ? mean[c] / n : 0.0;
+  for (int32_t i = 0; i < n; i++)
+    for (int c = 0; c < 3; c++) p[4 * (int64_t)i + c] -= mean[c];
+  return LJGPU_OK;
+}
+
 int ljgpu_validate_csr(int32_t n, const int32_t* nop, const int32_t* ptr,
                        const int32_t* list, int64_t n_entries, int kind) {
   return validate_csr(n, nop, ptr, list, n_entries, kind);

@@ -196,35 +196,81 @@ __global__ __launch_bounds__(kBlock) void k_copy_rows(int32_t n, const int32_t* 
   }
 }
 
-// Stable sort of each segment [start[r], start[r] + len[r]) of src into the
-// same positions of dst, by rank: element a goes to the number of elements
-// smaller than it plus the equal ones before it. One wavefront per segment,
-// the segment staged in LDS when it fits.
-constexpr int kSortLds = 2048;
+// Sort of each segment [start[r], start[r] + len[r]) of src into the same
+// positions of dst, ascending, one wavefront per segment. Segments of up to
+// 64*R elements are sorted in registers by a bitonic network (element
+// e = r*64 + lane in register r of that lane; partners closer than 64 are
+// exchanged across lanes, farther ones between a lane's own registers), padded
+// with INT32_MAX. Longer segments fall back to a rank sort from memory: each
+// element goes to the number of smaller elements plus the equal ones before
+// it.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(int32_t (&v)[R], int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int pr = r ^ (j / 64);
+          if (pr > r) {
+            const bool up = ((r * 64 + lane) & k) == 0;
+            const int32_t a = v[r], b = v[pr];
+            v[r] = up ? min(a, b) : max(a, b);
+            v[pr] = up ? max(a, b) : min(a, b);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int32_t other = __shfl_xor(v[r], j, 64);
+          const bool up = ((r * 64 + lane) & k) == 0;
+          const bool keep_min = ((lane & j) == 0) == up;
+          v[r] = keep_min ? min(v[r], other) : max(v[r], other);
+        }
+      }
+    }
+  }
+}
 
-__global__ __launch_bounds__(64) void k_row_sort(int32_t n, const int32_t* __restrict__ len,
-                                                 const int32_t* __restrict__ start,
-                                                 const int32_t* __restrict__ src,
-                                                 int32_t* __restrict__ dst) {
-  __shared__ int32_t sh[kSortLds];
-  const int lane = threadIdx.x;
-  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
+template <int R>
+__device__ __forceinline__ void sort_segment(const int32_t* __restrict__ src,
+                                             int32_t* __restrict__ dst, int32_t m, int lane) {
+  int32_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = r * 64 + lane;
+    v[r] = e < m ? src[e] : INT32_MAX;
+  }
+  wave_bitonic<R>(v, lane);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = r * 64 + lane;
+    if (e < m) dst[e] = v[r];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_sort(int32_t n, const int32_t* __restrict__ len,
+                                                     const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ src,
+                                                     int32_t* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6); r < n;
+       r += waves) {
     const int32_t m = len[r];
     const int64_t s = start[r];
-    if (m <= 0) continue;
-    if (m <= kSortLds) {
-      for (int32_t k = lane; k < m; k += 64) sh[k] = src[s + k];
-      __syncthreads();
-      for (int32_t a = lane; a < m; a += 64) {
-        const int32_t x = sh[a];
-        int32_t rank = 0;
-        for (int32_t b = 0; b < m; b++) {
-          const int32_t y = sh[b];
-          rank += (y < x) | ((y == x) & (b < a));
-        }
-        dst[s + rank] = x;
-      }
-      __syncthreads();
+    if (m <= 1) {
+      if (m == 1 && lane == 0) dst[s] = src[s];
+    } else if (m <= 64) {
+      sort_segment<1>(src + s, dst + s, m, lane);
+    } else if (m <= 128) {
+      sort_segment<2>(src + s, dst + s, m, lane);
+    } else if (m <= 256) {
+      sort_segment<4>(src + s, dst + s, m, lane);
+    } else if (m <= 512) {
+      sort_segment<8>(src + s, dst + s, m, lane);
     } else {
       for (int32_t a = lane; a < m; a += 64) {
         const int32_t x = src[s + a];
@@ -242,8 +288,7 @@ __global__ __launch_bounds__(64) void k_row_sort(int32_t n, const int32_t* __res
 int row_sort(hipStream_t s, int32_t n, const int32_t* len, const int32_t* start,
              const int32_t* src, int32_t* dst) {
   if (n <= 0) return LJGPU_OK;
-  const int grid = static_cast<int>(std::min<int64_t>(n, 1 << 20));
-  k_row_sort<<<grid, 64, 0, s>>>(n, len, start, src, dst);
+  k_row_sort<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(n, len, start, src, dst);
   LTRY(hipGetLastError(), "row sort kernel");
   return LJGPU_OK;
 }
@@ -478,25 +523,31 @@ __global__ __launch_bounds__(kBlock) void k_pairs(int32_t n, const double4* __re
 // ---------------------------------------------------------------------------
 // LDS tiles
 // ---------------------------------------------------------------------------
-constexpr int kTileHash = 8192;                        // open-addressing table slots
-constexpr int kTileHashLimit = 4096;                   // distinct keys before giving up
+constexpr int kTileHashBits = 12;
+constexpr int kTileHash = 1 << kTileHashBits;          // open-addressing table slots
+constexpr int kTileHashLimit = 3072;                   // distinct keys before giving up
 constexpr int kTileMaxStage = kTileLdsBudget / 24 + 1; // staged positions that can fit
+constexpr int kTileUniq = 4096;                        // sort buffer (power of two)
 static_assert(kTileMaxStage <= kTileHashLimit, "the hash limit must cover every tile that fits");
+static_assert(kTileHashLimit + kBlock < kTileHash, "probing must always find a free slot");
+static_assert(kTileHashLimit <= kTileUniq, "the sort buffer must hold every distinct key");
 
-// Distinct staged particles of a block: its own particles and every entry's
-// partner. Keys go into an LDS hash set, are compacted into uniq[] and
-// sorted ascending there. Returns the number of distinct keys, or -1 when
-// there are more than kTileHashLimit (the block cannot fit the LDS budget).
-__device__ int32_t tile_unique(int32_t i0, int32_t ni, const int32_t* __restrict__ list, int64_t e0,
-                               int32_t ne, int32_t* table, int32_t* uniq, int32_t* counter,
-                               int64_t* sh) {
+__device__ __forceinline__ uint32_t tile_hash(int32_t key) {
+  return (static_cast<uint32_t>(key) * 2654435761u) >> (32 - kTileHashBits);
+}
+
+// Insert the block's keys (its own particles and every entry's partner) into
+// an LDS hash set. Returns the number of distinct keys, or -1 when there are
+// more than kTileHashLimit (such a block cannot fit the LDS budget).
+__device__ int32_t tile_insert(int32_t i0, int32_t ni, const int32_t* __restrict__ list, int64_t e0,
+                               int32_t ne, int32_t* table, int32_t* counter) {
   for (int32_t t = threadIdx.x; t < kTileHash; t += kBlock) table[t] = -1;
   if (threadIdx.x == 0) *counter = 0;
   __syncthreads();
   const int64_t m = ni + static_cast<int64_t>(ne);
   for (int64_t t = threadIdx.x; t < m; t += kBlock) {
     const int32_t key = t < ni ? i0 + static_cast<int32_t>(t) : list[e0 + (t - ni)];
-    uint32_t h = (static_cast<uint32_t>(key) * 2654435761u) >> (32 - 13);
+    uint32_t h = tile_hash(key);
     while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <=
            kTileHashLimit) {
       const int32_t prev = atomicCAS(&table[h], -1, key);
@@ -510,36 +561,15 @@ __device__ int32_t tile_unique(int32_t i0, int32_t ni, const int32_t* __restrict
   }
   __syncthreads();
   const int32_t ns = *counter;
-  if (ns > kTileHashLimit) {
-    __syncthreads();
-    return -1;
-  }
-  // Compact the occupied slots (32 per thread) into uniq[0..ns).
-  constexpr int32_t C = kTileHash / kBlock;
-  int32_t cnt = 0;
-  for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++) cnt += table[t] != -1;
-  int64_t total;
-  int64_t off = block_exclusive(cnt, &total, sh);
-  for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++)
-    if (table[t] != -1) uniq[off++] = table[t];
-  int32_t P = 1;
-  while (P < ns) P <<= 1;
-  for (int32_t t = ns + threadIdx.x; t < P; t += kBlock) uniq[t] = INT32_MAX;
   __syncthreads();
-  for (int32_t k = 2; k <= P; k <<= 1)
-    for (int32_t j = k >> 1; j > 0; j >>= 1) {
-      for (int32_t t = threadIdx.x; t < P / 2; t += kBlock) {
-        const int32_t a = (t / j) * 2 * j + (t % j), b = a + j;
-        const bool up = (a & k) == 0;
-        const int32_t x = uniq[a], y = uniq[b];
-        if ((x > y) == up) {
-          uniq[a] = y;
-          uniq[b] = x;
-        }
-      }
-      __syncthreads();
-    }
-  return ns;
+  return ns > kTileHashLimit ? -1 : ns;
+}
+
+// Slot of a key known to be in the table.
+__device__ __forceinline__ int32_t tile_find(const int32_t* table, int32_t key) {
+  uint32_t h = tile_hash(key);
+  while (table[h] != key) h = (h + 1) & (kTileHash - 1);
+  return static_cast<int32_t>(h);
 }
 
 struct TileGeom {
@@ -567,12 +597,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int
                                                        int32_t* __restrict__ ns_out,
                                                        int32_t* __restrict__ stats) {
   __shared__ int32_t table[kTileHash];
-  __shared__ int32_t uniq[kTileHashLimit];
   __shared__ int32_t counter;
-  __shared__ int64_t sh[kBlock / 64];
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const TileGeom g = tile_geom(b, T, n, nop, ptr);
-    const int32_t ns = tile_unique(g.i0, g.ni, list, g.e0, g.ne, table, uniq, &counter, sh);
+    const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
     if (threadIdx.x == 0) {
       ns_out[b] = ns < 0 ? 0 : ns;
       const int64_t bytes = ns < 0 ? INT32_MAX : tile_lds_bytes(ns, g.ne);
@@ -580,11 +608,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int
       atomicMax(&stats[1], ns);
       atomicMax(&stats[2], static_cast<int32_t>(std::min<int64_t>(bytes, INT32_MAX)));
     }
-    __syncthreads();
   }
 }
 
-// Pass 2: descriptors, staged indices and per-entry LDS offsets.
+// Pass 2 (every block fits): the distinct keys sorted ascending become the
+// block's staged particles; each entry's LDS offset is 24 * the rank of its
+// partner, looked up through the hash table.
 __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int32_t nblocks,
                                                       const int32_t* __restrict__ nop,
                                                       const int32_t* __restrict__ ptr,
@@ -594,14 +623,44 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int3
                                                       int32_t* __restrict__ stage,
                                                       uint16_t* __restrict__ local) {
   __shared__ int32_t table[kTileHash];
-  __shared__ int32_t uniq[kTileHashLimit];
+  __shared__ uint16_t rank_of[kTileHash];
+  __shared__ int32_t uniq[kTileUniq];
   __shared__ int32_t counter;
   __shared__ int64_t sh[kBlock / 64];
+  constexpr int32_t C = kTileHash / kBlock;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const TileGeom g = tile_geom(b, T, n, nop, ptr);
-    const int32_t ns = tile_unique(g.i0, g.ni, list, g.e0, g.ne, table, uniq, &counter, sh);
+    const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
+    // Compact the occupied slots (C per thread) into uniq[0..ns), pad, sort.
+    int32_t cnt = 0;
+    for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++) cnt += table[t] != -1;
+    int64_t total;
+    int64_t off = block_exclusive(cnt, &total, sh);
+    for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++)
+      if (table[t] != -1) uniq[off++] = table[t];
+    int32_t P = 1;
+    while (P < ns) P <<= 1;
+    for (int32_t t = ns + threadIdx.x; t < P; t += kBlock) uniq[t] = INT32_MAX;
+    __syncthreads();
+    for (int32_t k = 2; k <= P; k <<= 1)
+      for (int32_t j = k >> 1; j > 0; j >>= 1) {
+        for (int32_t t = threadIdx.x; t < P / 2; t += kBlock) {
+          const int32_t a = (t / j) * 2 * j + (t % j), c = a + j;
+          const bool up = (a & k) == 0;
+          const int32_t x = uniq[a], y = uniq[c];
+          if ((x > y) == up) {
+            uniq[a] = y;
+            uniq[c] = x;
+          }
+        }
+        __syncthreads();
+      }
     const int64_t s0 = soff[b];
-    for (int32_t t = threadIdx.x; t < ns; t += kBlock) stage[s0 + t] = uniq[t];
+    for (int32_t t = threadIdx.x; t < ns; t += kBlock) {
+      const int32_t key = uniq[t];
+      stage[s0 + t] = key;
+      rank_of[tile_find(table, key)] = static_cast<uint16_t>(t);
+    }
     if (threadIdx.x == 0) {
       int32_t* d = desc + static_cast<int64_t>(b) * 8;
       d[0] = g.i0;
@@ -613,15 +672,10 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int3
       d[6] = 0;
       d[7] = 0;
     }
+    __syncthreads();
     for (int32_t e = threadIdx.x; e < g.ne; e += kBlock) {
       const int32_t j = list[g.e0 + e];
-      int32_t lo = 0, hi = ns;
-      while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (uniq[mid] < j) lo = mid + 1;
-        else hi = mid;
-      }
-      local[g.e0 + e] = static_cast<uint16_t>(lo * 24);
+      local[g.e0 + e] = static_cast<uint16_t>(rank_of[tile_find(table, j)] * 24);
     }
     __syncthreads();
   }

@@ -1,0 +1,443 @@
+// md.hip — molecular dynamics around the force path: the pair potential
+// energy over the installed list, and a velocity-Verlet driver (drift, force,
+// periodic list rebuild on the device, energy samples) for unit masses.
+//
+// The reference has no integrator and no energy: it only repeats force
+// evaluations on fixed positions (measure()'s loop, cuda/force_cuda.cu:
+// 331-335). The driver reuses that same force evaluation (p += F(q) * dt) as
+// its kick, so every step runs the library's hot path unchanged:
+//
+//   p += F(q) dt/2                          (t = 0 -> 1/2)
+//   for s = 1 .. steps:
+//     q += p dt, wrapped into [0, box)       (drift)
+//     rebuild the list every rebuild_every steps
+//     p += F(q) dt                           (kick, t = s-1/2 -> s+1/2)
+//   with the kick split into two halves around an energy sample at t = s,
+//   and a final half kick, so p is at integer time when the run ends.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "context.h"
+
+namespace ljgpu {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kEnergyBlocks = 1024;  // fixed grid: partial sums reduce in a fixed order
+
+__device__ __forceinline__ double wrap(double x, double box) {
+  x -= box * floor(x / box);
+  return x >= box ? x - box : x;  // rounding can land exactly on box
+}
+
+__global__ void k_drift(int32_t n, double* __restrict__ q, const double* __restrict__ p, double dt,
+                        double box) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double2 xy = reinterpret_cast<double2*>(q + 4 * i)[0];
+    double z = q[4 * i + 2];
+    const double2 pxy = reinterpret_cast<const double2*>(p + 4 * i)[0];
+    xy.x += pxy.x * dt;
+    xy.y += pxy.y * dt;
+    z += p[4 * i + 2] * dt;
+    if (box > 0.0) {
+      xy.x = wrap(xy.x, box);
+      xy.y = wrap(xy.y, box);
+      z = wrap(z, box);
+    }
+    reinterpret_cast<double2*>(q + 4 * i)[0] = xy;
+    q[4 * i + 2] = z;
+  }
+}
+
+__device__ __forceinline__ double image(double d, double box) {
+  return box > 0.0 ? d - box * rint(d / box) : d;
+}
+
+// Largest displacement |q - qref| (minimum image), as the bit pattern of a
+// non-negative double, whose integer order is the numeric order.
+__global__ void k_displacement(int32_t n, const double* __restrict__ q,
+                               const double* __restrict__ qref, double box,
+                               unsigned long long* __restrict__ out) {
+  double m = 0.0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double dx = image(q[4 * i] - qref[4 * i], box);
+    const double dy = image(q[4 * i + 1] - qref[4 * i + 1], box);
+    const double dz = image(q[4 * i + 2] - qref[4 * i + 2], box);
+    m = fmax(m, sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kBlock / 64; w++) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+// Potential energy: sum over the list of 4 (r^-12 - r^-6) for r2 <= cl2
+// (the force kernels' cutoff rule), and the number of those pairs, times
+// `weight` (1/2 for a full list, which holds every pair twice). One partial
+// sum of each per workgroup.
+__global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* __restrict__ nop,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ list,
+                                                      const double* __restrict__ q, double cl2,
+                                                      double box, double weight,
+                                                      double* __restrict__ partial,
+                                                      double* __restrict__ partial_count) {
+  __shared__ double sh[kBlock / 64];
+  double u = 0.0, cnt = 0.0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double xi = q[4 * i], yi = q[4 * i + 1], zi = q[4 * i + 2];
+    const int64_t kp = ptr[i];
+    double ui = 0.0;
+    for (int32_t k = 0; k < nop[i]; k++) {
+      const int64_t j = list[kp + k];
+      const double dx = image(q[4 * j] - xi, box);
+      const double dy = image(q[4 * j + 1] - yi, box);
+      const double dz = image(q[4 * j + 2] - zi, box);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > cl2) continue;
+      const double s = 1.0 / r2;
+      const double s3 = s * s * s;
+      ui += 4.0 * s3 * (s3 - 1.0);
+      cnt += 1.0;
+    }
+    u += ui;
+  }
+  const double t = block_sum(u * weight, sh);
+  const double c = block_sum(cnt * weight, sh);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = t;
+    partial_count[blockIdx.x] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_kinetic(int32_t n, const double* __restrict__ p,
+                                                    double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  double k = 0.0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double px = p[4 * i], py = p[4 * i + 1], pz = p[4 * i + 2];
+    k += 0.5 * (px * px + py * py + pz * pz);
+  }
+  const double t = block_sum(k, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+int grid_for(int64_t n) {
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65535)));
+}
+
+// U and K of the current state (synchronises; partial sums added in a fixed
+// order, so the result does not depend on scheduling).
+int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2, double* u,
+           double* k, double* pairs) {
+  *u = 0.0;
+  *k = 0.0;
+  *pairs = 0.0;
+  if (ctx->n == 0) return LJGPU_OK;
+  TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
+  double* part = ctx->md_red.as<double>();
+  const int grid = std::min(kEnergyBlocks, grid_for(ctx->n));
+  if (q) {
+    const double weight = ctx->eval_half ? 1.0 : 0.5;
+    k_potential<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, ctx->eval.nop.as<int32_t>(),
+                                                  ctx->eval.ptr.as<int32_t>(),
+                                                  ctx->eval.list.as<int32_t>(), q, cl2, box,
+                                                  weight, part, part + 2 * kEnergyBlocks);
+  }
+  if (p) k_kinetic<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, p, part + kEnergyBlocks);
+  HIP_TRY(hipGetLastError(), "energy kernels");
+  std::vector<double> h(3 * kEnergyBlocks);
+  HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                         ctx->stream), "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  for (int b = 0; b < grid; b++) {
+    if (q) *u += h[b];
+    if (p) *k += h[kEnergyBlocks + b];
+    if (q) *pairs += h[2 * kEnergyBlocks + b];
+  }
+  return LJGPU_OK;
+}
+
+}  // namespace
+
+}  // namespace ljgpu
+
+using namespace ljgpu;
+
+extern "C" {
+
+int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
+                 double* potential, double* kinetic, int64_t* pairs_in_cutoff) {
+  if (!ctx || !potential || !kinetic) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
+  if (!(cl2 >= 0.0)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "cl2 must be >= 0");
+  TRY(use_device(ctx));
+  double pairs = 0.0;
+  TRY(energy(ctx, q, p, box, cl2, potential, kinetic, &pairs));
+  if (pairs_in_cutoff) *pairs_in_cutoff = static_cast<int64_t>(pairs);
+  return LJGPU_OK;
+}
+
+int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
+                 const ljgpu_md_config* cfg, int32_t steps, ljgpu_md_stats* stats,
+                 double* samples, int32_t sample_capacity) {
+  if (!ctx || !cfg || !stats) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  if (n < 0 || steps < 0 || (n > 0 && (!q || !p)))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: bad particle arrays or steps");
+  if ((reinterpret_cast<uintptr_t>(q) | reinterpret_cast<uintptr_t>(p)) & 15u)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q and p must be 16-byte aligned");
+  if (!(cfg->dt > 0.0) || !std::isfinite(cfg->dt) || !(cfg->cutoff > 0.0) || !(cfg->skin >= 0.0) ||
+      cfg->rebuild_every < 0 || cfg->sample_every < 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: bad configuration");
+  if (cfg->precision != LJGPU_PREC_F64 && cfg->precision != LJGPU_PREC_MIXED)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: unknown precision");
+  if (cfg->accum != LJGPU_ACCUM_F64 && cfg->accum != LJGPU_ACCUM_KAHAN_F32)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: unknown accumulator");
+  if (samples && sample_capacity < 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: negative sample capacity");
+  std::memset(stats, 0, sizeof(*stats));
+  TRY(use_device(ctx));
+  const double dt = cfg->dt, cl2 = cfg->cutoff * cfg->cutoff;
+  const double search = cfg->cutoff + cfg->skin;
+  const bool mixed = cfg->precision == LJGPU_PREC_MIXED;
+  const int accum = cfg->accum;
+  PosIn in;
+  in.x = q;
+  in.y = q + 1;
+  in.z = q + 2;
+  in.stride = 4;
+  hipEvent_t e0 = nullptr, e1 = nullptr, b0 = nullptr, b1 = nullptr;
+  auto cleanup = [&]() {
+    for (hipEvent_t e : {e0, e1, b0, b1})
+      if (e) (void)hipEventDestroy(e);
+  };
+  struct Guard {
+    std::function<void()> f;
+    ~Guard() { f(); }
+  } guard{cleanup};
+  HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&b0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&b1), "hipEventCreate");
+  HIP_TRY(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+
+  float rebuild_ms = 0.0f;
+  int32_t rebuilds = 0, stale = 0;
+  double max_disp = 0.0;
+  TRY(dev_reserve(ctx->md_qref, sizeof(double) * 4 * std::max(n, 1), ctx->stream));
+  auto rebuild = [&]() -> int {
+    HIP_TRY(hipEventRecord(b0, ctx->stream), "hipEventRecord");
+    TRY(build_list(ctx, in, n, box, search, LJGPU_LIST_FULL));
+    if (n > 0)
+      HIP_TRY(hipMemcpyAsync(ctx->md_qref.p, q, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice,
+                             ctx->stream), "hipMemcpyAsync");
+    HIP_TRY(hipEventRecord(b1, ctx->stream), "hipEventRecord");
+    HIP_TRY(hipEventSynchronize(b1), "hipEventSynchronize");
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b0, b1), "hipEventElapsedTime");
+    rebuild_ms += ms;
+    rebuilds++;
+    return LJGPU_OK;
+  };
+  // Largest displacement since the last build (synchronises).
+  auto displacement = [&](double* out) -> int {
+    *out = 0.0;
+    if (n == 0) return LJGPU_OK;
+    TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
+    unsigned long long* slot =
+        reinterpret_cast<unsigned long long*>(ctx->md_red.as<double>() + 3 * kEnergyBlocks);
+    HIP_TRY(hipMemsetAsync(slot, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
+    k_displacement<<<std::min(1024, grid_for(n)), kBlock, 0, ctx->stream>>>(
+        n, q, ctx->md_qref.as<double>(), box, slot);
+    HIP_TRY(hipGetLastError(), "displacement kernel");
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    std::memcpy(out, &bits, sizeof(double));
+    return LJGPU_OK;
+  };
+
+  TRY(rebuild());
+  ForceArgs fa;
+  fa.soa = false;
+  fa.q[0] = q;
+  fa.p[0] = p;
+  fa.cl2 = cl2;
+  fa.box = box;
+  int32_t nsamples = 0;
+  double e_first = 0.0, max_drift = 0.0;
+  // The conserved energy of truncated forces is K plus the shifted potential
+  // (each pair inside the cutoff offset by -U(cutoff)), which is continuous
+  // where pairs cross the cutoff.
+  const double s6 = 1.0 / (cl2 * cl2 * cl2);
+  const double u_cut = 4.0 * s6 * (s6 - 1.0);
+  auto sample = [&](int32_t step) -> int {
+    double u = 0.0, k = 0.0, pairs = 0.0;
+    TRY(energy(ctx, q, p, box, cl2, &u, &k, &pairs));
+    u -= pairs * u_cut;
+    if (nsamples == 0) {
+      stats->potential0 = u;
+      stats->kinetic0 = k;
+      e_first = u + k;
+    }
+    stats->potential1 = u;
+    stats->kinetic1 = k;
+    max_drift = std::max(max_drift, std::fabs(u + k - e_first));
+    if (samples && nsamples < sample_capacity) {
+      samples[4 * nsamples] = step;
+      samples[4 * nsamples + 1] = u;
+      samples[4 * nsamples + 2] = k;
+      samples[4 * nsamples + 3] = u + k;
+    }
+    nsamples++;
+    return LJGPU_OK;
+  };
+  auto kick = [&](double h) -> int {
+    fa.dt = h;
+    TRY(enqueue_force(ctx, fa, mixed, accum));
+    stats->pair_entries += ctx->eval.e;
+    return LJGPU_OK;
+  };
+  auto drift = [&]() -> int {
+    if (n == 0) return LJGPU_OK;
+    k_drift<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, q, p, dt, box);
+    HIP_TRY(hipGetLastError(), "drift kernel");
+    return LJGPU_OK;
+  };
+
+  TRY(sample(0));
+  if (steps > 0) TRY(kick(0.5 * dt));
+  hipGraphExec_t seg_exec = nullptr;
+  int32_t seg_len = 0;
+  StepKey seg_key{};
+  auto drop_seg = [&]() {
+    if (seg_exec) (void)hipGraphExecDestroy(seg_exec);
+    seg_exec = nullptr;
+    seg_len = 0;
+  };
+  struct SegGuard {
+    std::function<void()> f;
+    ~SegGuard() { f(); }
+  } seg_guard{drop_seg};
+
+  // rebuild_every = 0: rebuild whenever some particle has moved more than
+  // skin/2 since the last build (checked after every drift).
+  const bool adaptive = cfg->rebuild_every == 0;
+  int32_t s = 1;
+  while (s <= steps) {
+    const bool rebuild_now = !adaptive && s % cfg->rebuild_every == 0;
+    const bool sample_now = (cfg->sample_every > 0 && s % cfg->sample_every == 0) || s == steps;
+    if (adaptive || rebuild_now || sample_now || !ctx->use_graphs) {
+      TRY(drift());
+      if (adaptive) {
+        double d = 0.0;
+        TRY(displacement(&d));
+        if (2.0 * d > cfg->skin) {
+          TRY(rebuild());
+          d = 0.0;
+        }
+        max_disp = std::max(max_disp, d);
+      } else if (rebuild_now) {
+        double d = 0.0;
+        TRY(displacement(&d));
+        max_disp = std::max(max_disp, d);
+        if (2.0 * d > cfg->skin) stale++;
+        TRY(rebuild());
+      }
+      if (sample_now) {
+        TRY(kick(0.5 * dt));
+        TRY(sample(s));
+        if (s < steps) TRY(kick(0.5 * dt));
+      } else {
+        TRY(kick(dt));
+      }
+      s++;
+      continue;
+    }
+    // Plain steps up to the next rebuild or sample run as one captured
+    // graph of (drift, kick) pairs, re-captured when its arguments change.
+    int32_t len = 0;
+    while (s + len <= steps && len < kGraphChunk) {
+      const int32_t t = s + len;
+      if (t % cfg->rebuild_every == 0 || (cfg->sample_every > 0 && t % cfg->sample_every == 0) ||
+          t == steps)
+        break;
+      len++;
+    }
+    if (len < kGraphMinSteps) {
+      for (int32_t t = 0; t < len; t++) {
+        TRY(drift());
+        TRY(kick(dt));
+      }
+      s += len;
+      continue;
+    }
+    fa.dt = dt;
+    StepKey key;
+    std::memset(&key, 0, sizeof(key));
+    TRY(force_view(ctx, &key.lv, &key.lc));
+    key.fa = fa;
+    key.mixed = mixed;
+    key.accum = accum;
+    key.steps = len;
+    if (!seg_exec || seg_len != len || std::memcmp(&key, &seg_key, sizeof(key)) != 0) {
+      drop_seg();
+      TRY(capture(ctx, [&]() {
+        for (int32_t t = 0; t < len; t++) {
+          TRY(drift());
+          TRY(enqueue_force(ctx, fa, mixed, accum));
+        }
+        return LJGPU_OK;
+      }, &seg_exec));
+      seg_len = len;
+      seg_key = key;
+    }
+    HIP_TRY(hipGraphLaunch(seg_exec, ctx->stream), "hipGraphLaunch");
+    stats->pair_entries += static_cast<int64_t>(len) * ctx->eval.e;
+    s += len;
+  }
+
+  HIP_TRY(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+  HIP_TRY(hipEventSynchronize(e1), "hipEventSynchronize");
+  float total_ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&total_ms, e0, e1), "hipEventElapsedTime");
+  stats->steps = steps;
+  stats->rebuilds = rebuilds;
+  stats->samples = nsamples;
+  stats->max_abs_drift = max_drift;
+  stats->max_displacement = max_disp;
+  stats->stale_lists = stale;
+  stats->ms_total = total_ms;
+  stats->ms_rebuild = rebuild_ms;
+  return LJGPU_OK;
+}
+
+int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (enable != 0 && enable != 1) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "enable must be 0 or 1");
+  ctx->use_graphs = enable != 0;
+  if (!ctx->use_graphs) drop_graph(ctx);
+  return LJGPU_OK;
+}
+
+}  // extern "C"

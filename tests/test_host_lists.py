@@ -131,3 +131,16 @@ def test_make_transposed_matches_oracle():
     np.testing.assert_array_equal(G.make_transposed(full, w), O.make_transposed(ocsr, w))
     with pytest.raises(G.LJGPUError):
         G.make_transposed(full, w - 3)
+
+
+def test_maxwell_boltzmann_statistics():
+    """Seeded, zero mean momentum, per-component variance T."""
+    n, t = 200_000, 1.3
+    p = G.maxwell_boltzmann(n, t, 11)
+    assert p.shape == (n, 4) and np.all(p[:, 3] == 0.0)
+    np.testing.assert_array_equal(p, G.maxwell_boltzmann(n, t, 11))
+    assert not np.array_equal(p, G.maxwell_boltzmann(n, t, 12))
+    assert np.abs(p[:, :3].mean(axis=0)).max() < 1e-12
+    var = p[:, :3].var(axis=0)
+    # Sample variance of n normal draws: relative standard error sqrt(2/n).
+    assert np.abs(var / t - 1.0).max() < 5 * np.sqrt(2.0 / n)
