@@ -156,6 +156,19 @@ int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t particle_number, d
 int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
                          int32_t particle_number, double box, double search_length, int kind);
 
+/* Batched replicas: n_replicas independent systems stored back to back,
+ * replica r holding particles [offsets[r], offsets[r+1]) in a cubic box of
+ * side boxes[r] (periodic for boxes[r] > 0, open otherwise); host arrays,
+ * offsets[0] = 0. Afterwards the list (ljgpu_set_list_* with
+ * particle_number = offsets[n_replicas], every pair inside one replica, or
+ * ljgpu_build_list with box = 0, built per replica) and every force call
+ * (box = 0) cover all replicas in one launch of the tiled kernel: its tiles
+ * never straddle two replicas and each carries its replica's box. Replaces
+ * the installed list; n_replicas = 0 returns to a single system. No
+ * reference counterpart (the reference runs one system). */
+int ljgpu_set_replicas(ljgpu_ctx* ctx, int32_t n_replicas, const int32_t* offsets,
+                       const double* boxes);
+
 /* Half-list evaluation mode (LJGPU_HALF_*); takes effect on the next
  * ljgpu_set_list_* call. */
 int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode);
@@ -350,7 +363,8 @@ int ljgpu_make_transposed(int32_t particle_number, const int32_t* number_of_part
  * number_of_partners[i], as sortpair produces): the tiled counterpart of
  * make_transposed_pairlist, cuda/force_cuda.cu:229-240. desc receives 8 ints
  * per block (first particle, particle count, staging offset, staging count,
- * first list entry, entry count, 0, 0), stage the staged particle indices,
+ * first list entry, entry count, replica (0 here), 0), stage the staged
+ * particle indices,
  * local (n_entries values, parallel to sorted_list) the LDS byte offset
  * 24*slot of each entry's j within its block's staging area; max_stage the
  * largest staging count. Pass null outputs to query the sizes first. */

@@ -151,6 +151,7 @@ def lib() -> C.CDLL:
         "ljgpu_build_list_soa": [_vp, _vp, _vp, _vp, _i32, _d, _d, C.c_int],
         "ljgpu_get_tiles": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i32),
                             C.POINTER(_i64)],
+        "ljgpu_set_replicas": [_vp, _i32, _vp, _vp],
         "ljgpu_md_run": [_vp, _vp, _vp, _i32, _d, C.POINTER(MDConfig), _i32, C.POINTER(MDStats),
                          _vp, _i32],
         "ljgpu_energy": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
@@ -278,6 +279,26 @@ def make_transposed(csr: CSR, width: int | None = None) -> np.ndarray:
     _check(lib().ljgpu_make_transposed(csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer),
                                        _ptr(csr.sorted_list), width, _ptr(out)))
     return out[: csr.n * width]
+
+
+def replica_workload(n_replicas: int = 256, seed: int = 4, cells: tuple[int, int] = (6, 20),
+                     density: tuple[float, float] = (0.5, 1.0)):
+    """Batched-replica workload: replica b is a periodic FCC lattice of
+    4*k_b^3 particles at density rho_b (k_b uniform in cells, inclusive;
+    rho_b uniform in density, both drawn from numpy's default_rng(seed)),
+    jittered with seed 1000 + b. Returns (q (N, 4), offsets, boxes)."""
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(cells[0], cells[1] + 1, size=n_replicas)
+    rhos = rng.uniform(density[0], density[1], size=n_replicas)
+    qs, boxes = [], []
+    for b in range(n_replicas):
+        q, box = lattice_periodic(int(ks[b]), float(rhos[b]), 1000 + b)
+        qs.append(q)
+        boxes.append(box)
+    offsets = np.zeros(n_replicas + 1, np.int32)
+    offsets[1:] = np.cumsum([q.shape[0] for q in qs])
+    q = np.concatenate(qs) if qs else np.zeros((0, 4))
+    return q, offsets, np.array(boxes, np.float64)
 
 
 def maxwell_boltzmann(n: int, temperature: float, seed: int) -> np.ndarray:
@@ -498,6 +519,15 @@ class Context:
                                      _ptr(local), local.size, C.byref(nb), C.byref(ns)))
         d = desc.reshape(-1, 8)
         return Tiles(d, stage[: ns.value], local[:e], int(d[:, 3].max()))
+
+    def set_replicas(self, offsets, boxes) -> None:
+        """Batched replicas (offsets: n_replicas + 1 particle offsets; boxes:
+        one side per replica); empty arguments return to one system."""
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64)
+        nrep = max(off.size - 1, 0)
+        _check(lib().ljgpu_set_replicas(self.handle, nrep, _ptr(off) if nrep else None,
+                                        _ptr(bx) if nrep else None))
 
     def set_graphs(self, enable: bool) -> None:
         _check(lib().ljgpu_set_graphs(self.handle, 1 if enable else 0))

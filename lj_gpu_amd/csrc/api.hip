@@ -74,7 +74,7 @@ int ensure_tiles(ljgpu_ctx* ctx) {
   ctx->tiles.tile_particles = 0;
   ctx->tiles.blocks = 0;
   for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
-    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->work, ctx->tiles));
+    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, ctx->work, ctx->tiles));
     if (ctx->tiles.tile_particles > 0) break;
   }
   ctx->tiles_for = want;
@@ -104,6 +104,13 @@ int derive_list(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* pt
     // Rows stored in order make every tile's entries one contiguous range.
     TRY(dev_copy_row_ordered(ctx->stream, n, nop, ptr, list, e, ctx->work, ctx->eval));
     ctx->eval_half = (kind == LJGPU_LIST_HALF);
+  }
+  if (!ctx->rep_off.empty()) {
+    const int rc = dev_check_replicas(ctx->stream, ctx->eval, ctx->rep_off, ctx->work);
+    if (rc) {
+      reset_list(ctx);
+      return rc;
+    }
   }
   ctx->ell_ready = false;
   ctx->tiles_for = -1;
@@ -153,14 +160,30 @@ int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double sea
                      "build_list: periodic box shorter than twice the search length");
   if (n > 0 && (!q.x || !q.y || !q.z))
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "build_list: null positions");
+  if (!ctx->rep_off.empty()) {
+    if (n != ctx->rep_off.back())
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                       "build_list: particle_number differs from the replicas' total");
+    if (box != 0.0)
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                       "build_list: with replicas, pass box = 0 (each replica has its own)");
+    for (double b : ctx->rep_boxes)
+      if (b > 0.0 && b < 2.0 * search)
+        return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                         "build_list: a replica box is shorter than twice the search length");
+  }
   TRY(use_device(ctx));
   reset_list(ctx);
   const bool expand = kind == LJGPU_LIST_FULL || ctx->half_mode_req != LJGPU_HALF_ATOMIC;
   // A full list is the symmetric closure of the half list (the pair
   // predicate is exactly symmetric), so an expanded half list is built
   // directly as the full list.
-  TRY(dev_makepair(ctx->stream, q, n, box, search, expand ? LJGPU_LIST_FULL : LJGPU_LIST_HALF,
-                   ctx->work, ctx->eval));
+  const int build_kind = expand ? LJGPU_LIST_FULL : LJGPU_LIST_HALF;
+  if (ctx->rep_off.empty())
+    TRY(dev_makepair(ctx->stream, q, n, box, search, build_kind, ctx->work, ctx->eval));
+  else
+    TRY(dev_makepair_batch(ctx->stream, q, ctx->rep_off, ctx->rep_boxes, search, build_kind,
+                           ctx->work, ctx->eval));
   ctx->n = n;
   ctx->kind = kind;
   ctx->eval_half = !expand;
@@ -180,9 +203,12 @@ int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double sea
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps) {
+int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps, double box) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  if (!ctx->rep_off.empty() && box != 0.0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "with replicas, pass box = 0 (each replica has its own box)");
   if (steps < 0) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "steps < 0");
   if (!(cl2 >= 0.0)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "cl2 must be >= 0");
   if (dt != dt) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "dt is NaN");
@@ -206,6 +232,7 @@ int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc) {
   lc->cus = ctx->cus;
   lc->half_atomic = ctx->eval_half;
   lc->clamp = false;
+  if (!ctx->rep_off.empty()) lv->replica_box = ctx->rep_box_dev.as<double>();
   if (ctx->lanes == 1) {
     lv->list = ctx->ell.as<int32_t>();
     lv->ell_width = ctx->ell_width;
@@ -347,6 +374,7 @@ int ljgpu_destroy(ljgpu_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   drop_graph(ctx);
   release_all(ctx);
+  dev_release(ctx->rep_box_dev);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -480,6 +508,9 @@ int ljgpu_set_list_csr(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int3
   if (n_entries > 0 && !list) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null sorted_list");
   if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "kind must be LJGPU_LIST_HALF or LJGPU_LIST_FULL");
+  if (!ctx->rep_off.empty() && n != ctx->rep_off.back())
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "particle_number differs from the replicas' total");
   if (mem == LJGPU_MEM_HOST) {
     int rc = validate_csr(n, nop, ptr, list, n_entries, kind);
     if (rc) return rc;
@@ -603,6 +634,39 @@ int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_
   return LJGPU_OK;
 }
 
+int ljgpu_set_replicas(ljgpu_ctx* ctx, int32_t n_replicas, const int32_t* offsets,
+                       const double* boxes) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (n_replicas < 0 || (n_replicas > 0 && (!offsets || !boxes)))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "set_replicas: bad argument");
+  if (n_replicas > 0 && offsets[0] != 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "set_replicas: offsets[0] must be 0");
+  for (int32_t r = 0; r < n_replicas; r++) {
+    if (offsets[r + 1] < offsets[r])
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT, "set_replicas: offsets must not decrease");
+    if (!(boxes[r] == boxes[r]))
+      return set_error(LJGPU_ERR_INVALID_ARGUMENT, "set_replicas: box is NaN");
+  }
+  TRY(use_device(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  drop_graph(ctx);
+  reset_list(ctx);
+  ctx->rep_off.clear();
+  ctx->rep_boxes.clear();
+  if (n_replicas == 0) return LJGPU_OK;
+  ctx->rep_off.assign(offsets, offsets + n_replicas + 1);
+  ctx->rep_boxes.assign(boxes, boxes + n_replicas);
+  std::vector<double> bb(2 * static_cast<size_t>(n_replicas));
+  for (int32_t r = 0; r < n_replicas; r++) {
+    bb[2 * r] = boxes[r] > 0.0 ? boxes[r] : 0.0;
+    bb[2 * r + 1] = boxes[r] > 0.0 ? 1.0 / boxes[r] : 0.0;
+  }
+  TRY(dev_reserve(ctx->rep_box_dev, sizeof(double) * bb.size(), ctx->stream));
+  HIP_TRY(hipMemcpy(ctx->rep_box_dev.p, bb.data(), sizeof(double) * bb.size(),
+                    hipMemcpyHostToDevice), "hipMemcpy");
+  return LJGPU_OK;
+}
+
 int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t n, double box,
                      double search_length, int kind) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
@@ -629,7 +693,7 @@ int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, con
 
 int ljgpu_force_aos_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
                         double box, int32_t steps) {
-  int rc = check_force_args(ctx, dt, cl2, steps);
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
   if (rc) return rc;
   if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
   if (!aligned16(q) || !aligned16(p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q and p must be 16-byte aligned");
@@ -646,7 +710,7 @@ int ljgpu_force_aos_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, d
 int ljgpu_force_soa_f64(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
                         double* px, double* py, double* pz, double dt, double cl2, double box,
                         int32_t steps) {
-  int rc = check_force_args(ctx, dt, cl2, steps);
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
   if (rc) return rc;
   if (ctx->n > 0 && (!qx || !qy || !qz || !px || !py || !pz))
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p component");
@@ -662,7 +726,7 @@ int ljgpu_force_soa_f64(ljgpu_ctx* ctx, const double* qx, const double* qy, cons
 
 int ljgpu_force_aos_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
                           double box, int32_t steps, int accum) {
-  int rc = check_force_args(ctx, dt, cl2, steps);
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
   if (rc) return rc;
   if (accum != LJGPU_ACCUM_F64 && accum != LJGPU_ACCUM_KAHAN_F32)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unknown accumulator");
@@ -681,7 +745,7 @@ int ljgpu_force_aos_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt,
 int ljgpu_force_soa_mixed(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
                           double* px, double* py, double* pz, double dt, double cl2, double box,
                           int32_t steps, int accum) {
-  int rc = check_force_args(ctx, dt, cl2, steps);
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
   if (rc) return rc;
   if (accum != LJGPU_ACCUM_F64 && accum != LJGPU_ACCUM_KAHAN_F32)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unknown accumulator");

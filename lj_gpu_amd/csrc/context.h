@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <functional>
 #include <string>
+#include <vector>
 
 #include "ljgpu_internal.h"
 #include "lists.h"
@@ -70,6 +71,12 @@ struct ljgpu_ctx {
   hipGraphExec_t graph_exec = nullptr;
   ljgpu::StepKey graph_key{};
   int32_t graph_steps = 0;
+
+  // Batched replicas: particle offsets (n_replicas + 1) and boxes on the
+  // host, (box, 1/box) per replica on the device; empty for one system.
+  std::vector<int32_t> rep_off;
+  std::vector<double> rep_boxes;
+  ljgpu::DevMem rep_box_dev;
 
   // Molecular-dynamics state: positions at the last list build, reduction
   // scratch.

@@ -342,12 +342,24 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
                                           const int4* __restrict__ desc,
                                           const int32_t* __restrict__ stage,
                                           const uint16_t* __restrict__ local, int b,
-                                          const PairParams& pp, double reach, char* lb,
+                                          const PairParams& gpp, double greach,
+                                          const double2* __restrict__ rep_box, char* lb,
                                           int* flag) {
   const int4 d0 = desc[2 * b];
   const int4 d1 = desc[2 * b + 1];
   const int i0 = d0.x, ni = d0.y, soff = d0.z, ns = d0.w;
   const int eoff = d1.x, ne = d1.y;
+  // Batched replicas: the block's own box (box, 1/box) from its replica
+  // (descriptor slot 6); an open-box replica has (0, 0), for which every
+  // minimum image is the identity.
+  PairParams pp = gpp;
+  double reach = greach;
+  if (PBC && rep_box) {
+    const double2 bb = rep_box[d1.z];
+    pp.box = bb.x;
+    pp.inv_box = bb.y;
+    reach = 0.5 * bb.x - sqrt(pp.cl2);
+  }
   double* lds = reinterpret_cast<double*>(lb);
   const int off_area = (ns * 24 + 15) & ~15;
   double cx = 0.0, cy = 0.0, cz = 0.0;
@@ -443,7 +455,8 @@ template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
 __global__ void __launch_bounds__(1024)
 k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
        const int4* __restrict__ desc, const int32_t* __restrict__ stage,
-       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach) {
+       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach,
+       const double2* __restrict__ rep_box) {
   // Dynamic LDS only (no static LDS, so its base is address 0 and constant
   // offsets fold into the LDS instructions): a 16-byte header holding the
   // frame-check word, then the tile's positions (and offsets).
@@ -458,7 +471,7 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   const int t_end = min(nblocks, (xcd + 1) * per);
   for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
     tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, ptr, desc, stage, local, b, pp,
-                                                 reach, lb, flag);
+                                                 reach, rep_box, lb, flag);
     __syncthreads();  // the next tile overwrites this tile's LDS
   }
 }
@@ -513,7 +526,7 @@ int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& 
   hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(grid), dim3(threads),
                      lds_bytes, s, pos, mom, lv.nop, lv.ptr,
                      reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
-                     lv.tile_blocks, pp, reach);
+                     lv.tile_blocks, pp, reach, reinterpret_cast<const double2*>(lv.replica_box));
   return launch_status("tiled force kernel launch");
 }
 
@@ -598,6 +611,14 @@ int run_layout(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams
     } else {
       return set_error(LJGPU_ERR_UNSUPPORTED, "clamped launch outside AoS FP64");
     }
+  }
+  if (lv.replica_box) {
+    // Batched replicas: tiles only, each block with its replica's box.
+    if (lv.tile_blocks <= 0 || lc.half_atomic)
+      return set_error(LJGPU_ERR_UNSUPPORTED,
+                       "batched replicas need the tiled kernel (lanes_per_particle > 1, "
+                       "tiling on, no atomic half list)");
+    return run_tile_lanes<PREC, P, M, true>(s, pos, mom, lv, pp, lc);
   }
   if (lv.tile_blocks > 0 && !lc.half_atomic) {
     return pbc ? run_tile_lanes<PREC, P, M, true>(s, pos, mom, lv, pp, lc)

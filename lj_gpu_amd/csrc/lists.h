@@ -9,6 +9,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "ljgpu_internal.h"
 
@@ -62,6 +63,9 @@ struct ListWork {
   DevMem tmp;         // int32 per list entry
   DevMem tile_ns;     // int32 per tile (+1)
   DevMem tile_soff;   // int32 per tile (+1)
+  DevMem tile_geo;    // int32 x 4 per tile: first particle, count, replica
+  DevMem rep_of;      // int32 per particle: replica index
+  DevCSR part;        // one replica's list during a batched build
 };
 void dev_release(ListWork& w);
 
@@ -111,10 +115,22 @@ int dev_half_row_stats(hipStream_t s, const DevCSR& full, ListWork& w, int32_t* 
 int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell);
 
 // LDS tiling of a row-ordered full list with blocks of `tile_particles`
-// consecutive particles. Returns LJGPU_OK with out.tile_particles = 0 when a
-// block does not fit the LDS budget (the caller then halves the tile size or
+// consecutive particles, restarted at each replica boundary of `replicas`
+// (host offsets; empty for a single system), descriptor slot 6 holding the
+// block's replica. Returns LJGPU_OK with out.tile_particles = 0 when a block
+// does not fit the LDS budget (the caller then halves the tile size or
 // evaluates untiled).
-int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles, ListWork& w,
-                   DevTiles& out);
+int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles,
+                   const std::vector<int32_t>& replicas, ListWork& w, DevTiles& out);
+
+// Batched replicas: lists of every replica (particles [off[r], off[r+1]),
+// box boxes[r]) built on the device into one CSR with global indices.
+int dev_makepair_batch(hipStream_t s, const PosIn& q, const std::vector<int32_t>& off,
+                       const std::vector<double>& boxes, double search_length, int kind,
+                       ListWork& w, DevCSR& out);
+
+// LJGPU_ERR_BAD_LIST unless every entry of row i lies in i's replica.
+int dev_check_replicas(hipStream_t s, const DevCSR& c, const std::vector<int32_t>& off,
+                       ListWork& w);
 
 }  // namespace ljgpu

@@ -573,16 +573,19 @@ __device__ __forceinline__ int32_t tile_find(const int32_t* table, int32_t key) 
 }
 
 struct TileGeom {
-  int32_t i0, ni, ne;
+  int32_t i0, ni, ne, rid;
   int64_t e0;
 };
 
-__device__ __forceinline__ TileGeom tile_geom(int32_t b, int32_t T, int32_t n,
+// Block b covers particles [geo[4b], geo[4b] + geo[4b+1]) of replica
+// geo[4b+2]; its entries are one contiguous range of the row-ordered list.
+__device__ __forceinline__ TileGeom tile_geom(int32_t b, const int32_t* __restrict__ geo,
                                               const int32_t* __restrict__ nop,
                                               const int32_t* __restrict__ ptr) {
   TileGeom g;
-  g.i0 = b * T;
-  g.ni = min(T, n - g.i0);
+  g.i0 = geo[4 * b];
+  g.ni = geo[4 * b + 1];
+  g.rid = geo[4 * b + 2];
   g.e0 = ptr[g.i0];
   g.ne = static_cast<int32_t>(static_cast<int64_t>(ptr[g.i0 + g.ni - 1]) + nop[g.i0 + g.ni - 1] - g.e0);
   return g;
@@ -590,7 +593,7 @@ __device__ __forceinline__ TileGeom tile_geom(int32_t b, int32_t T, int32_t n,
 
 // Pass 1: distinct staged particles per block; flags blocks that exceed the
 // LDS budget. stats: [0] overflow, [1] max staged, [2] max LDS bytes.
-__global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int32_t nblocks,
+__global__ __launch_bounds__(kBlock) void k_tile_count(const int32_t* __restrict__ geo, int32_t nblocks,
                                                        const int32_t* __restrict__ nop,
                                                        const int32_t* __restrict__ ptr,
                                                        const int32_t* __restrict__ list,
@@ -599,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int
   __shared__ int32_t table[kTileHash];
   __shared__ int32_t counter;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const TileGeom g = tile_geom(b, T, n, nop, ptr);
+    const TileGeom g = tile_geom(b, geo, nop, ptr);
     const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
     if (threadIdx.x == 0) {
       ns_out[b] = ns < 0 ? 0 : ns;
@@ -614,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(int32_t n, int32_t T, int
 // Pass 2 (every block fits): the distinct keys sorted ascending become the
 // block's staged particles; each entry's LDS offset is 24 * the rank of its
 // partner, looked up through the hash table.
-__global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int32_t nblocks,
+__global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict__ geo, int32_t nblocks,
                                                       const int32_t* __restrict__ nop,
                                                       const int32_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ list,
@@ -629,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int3
   __shared__ int64_t sh[kBlock / 64];
   constexpr int32_t C = kTileHash / kBlock;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const TileGeom g = tile_geom(b, T, n, nop, ptr);
+    const TileGeom g = tile_geom(b, geo, nop, ptr);
     const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
     // Compact the occupied slots (C per thread) into uniq[0..ns), pad, sort.
     int32_t cnt = 0;
@@ -669,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int3
       d[3] = ns;
       d[4] = static_cast<int32_t>(g.e0);
       d[5] = g.ne;
-      d[6] = 0;
+      d[6] = g.rid;
       d[7] = 0;
     }
     __syncthreads();
@@ -678,6 +681,43 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(int32_t n, int32_t T, int3
       local[g.e0 + e] = static_cast<uint16_t>(rank_of[tile_find(table, j)] * 24);
     }
     __syncthreads();
+  }
+}
+
+// Batched replicas.
+__global__ void k_copy_add(int64_t n, const int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                           int32_t add) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[t] = src[t] + add;
+}
+
+__global__ void k_replica_of(int32_t n, const int32_t* __restrict__ off, int32_t nrep,
+                             int32_t* __restrict__ rep) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int32_t lo = 0, hi = nrep;  // last r with off[r] <= t
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (off[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    rep[t] = lo;
+  }
+}
+
+__global__ void k_check_replicas(int32_t n, const int32_t* __restrict__ nop,
+                                 const int32_t* __restrict__ ptr, const int32_t* __restrict__ list,
+                                 const int32_t* __restrict__ rep,
+                                 unsigned long long* __restrict__ bad) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t kp = ptr[t];
+    for (int32_t k = 0; k < nop[t]; k++)
+      if (rep[list[kp + k]] != rep[t]) {
+        atomicMin(bad, static_cast<unsigned long long>(t));
+        break;
+      }
   }
 }
 
@@ -728,8 +768,10 @@ void dev_release(DevTiles& t) {
 
 void dev_release(ListWork& w) {
   for (DevMem* m : {&w.scan_sums, &w.small, &w.cell_count, &w.cell_start, &w.cell_slot, &w.cell_of,
-                    &w.cell_sorted, &w.cpos, &w.counts, &w.tmp, &w.tile_ns, &w.tile_soff})
+                    &w.cell_sorted, &w.cpos, &w.counts, &w.tmp, &w.tile_ns, &w.tile_soff,
+                    &w.tile_geo, &w.rep_of})
     dev_release(*m);
+  dev_release(w.part);
 }
 
 int dev_scan(hipStream_t s, const int32_t* in, int32_t* out, int64_t n, ListWork& w,
@@ -954,18 +996,31 @@ int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell) {
   return LJGPU_OK;
 }
 
-int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, ListWork& w, DevTiles& out) {
+int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<int32_t>& reps,
+                   ListWork& w, DevTiles& out) {
   out.tile_particles = 0;
   out.blocks = 0;
   if (T <= 0 || c.n <= 0) return LJGPU_OK;
-  const int32_t nb = (c.n + T - 1) / T;
+  // Blocks of T consecutive particles, restarted at every replica boundary.
+  std::vector<int32_t> geo;
+  const std::vector<int32_t> single = {0, c.n};
+  const std::vector<int32_t>& off = reps.empty() ? single : reps;
+  for (size_t r = 0; r + 1 < off.size(); r++)
+    for (int32_t i0 = off[r]; i0 < off[r + 1]; i0 += T)
+      geo.insert(geo.end(), {i0, std::min(T, off[r + 1] - i0), static_cast<int32_t>(r), 0});
+  const int32_t nb = static_cast<int32_t>(geo.size() / 4);
+  if (nb == 0) return LJGPU_OK;
+  LCHECK(dev_reserve(w.tile_geo, sizeof(int32_t) * geo.size(), s));
+  LTRY(hipMemcpyAsync(w.tile_geo.p, geo.data(), sizeof(int32_t) * geo.size(),
+                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  const int32_t* dgeo = w.tile_geo.as<int32_t>();
   LCHECK(dev_reserve(w.tile_ns, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
   LCHECK(dev_reserve(w.tile_soff, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
   LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
   int32_t* stats = w.small.as<int32_t>() + 4;
   LCHECK(fill<int32_t>(s, stats, 3, 0));
   const int grid = std::min(nb, 65535);
-  k_tile_count<<<grid, kBlock, 0, s>>>(c.n, T, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+  k_tile_count<<<grid, kBlock, 0, s>>>(dgeo, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
                                        c.list.as<int32_t>(), w.tile_ns.as<int32_t>(), stats);
   LTRY(hipGetLastError(), "tile count kernel");
   int32_t h[3];
@@ -981,7 +1036,7 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, ListWork& w, DevTi
   const int64_t nlocal = c.e + kTileLocalPad;
   LCHECK(dev_reserve(out.local, sizeof(uint16_t) * nlocal, s));
   LCHECK(fill<uint16_t>(s, out.local.as<uint16_t>() + c.e, kTileLocalPad, 0));
-  k_tile_fill<<<grid, kBlock, 0, s>>>(c.n, T, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+  k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
                                       c.list.as<int32_t>(), w.tile_soff.as<int32_t>(),
                                       out.desc.as<int32_t>(), out.stage.as<int32_t>(),
                                       out.local.as<uint16_t>());
@@ -992,6 +1047,79 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, ListWork& w, DevTi
   out.max_stage = h[1];
   out.lds_bytes = h[2];
   return LJGPU_OK;
+}
+
+int dev_makepair_batch(hipStream_t s, const PosIn& q, const std::vector<int32_t>& off,
+                       const std::vector<double>& boxes, double search_length, int kind,
+                       ListWork& w, DevCSR& out) {
+  const int32_t nrep = static_cast<int32_t>(off.size()) - 1;
+  const int32_t n = off.back();
+  out.n = n;
+  LCHECK(dev_reserve(out.nop, sizeof(int32_t) * std::max(n, 1), s));
+  LCHECK(dev_reserve(out.ptr, sizeof(int32_t) * (static_cast<size_t>(n) + 1), s));
+  auto part_of = [&](int32_t r) {
+    PosIn qr = q;
+    const int64_t shift = static_cast<int64_t>(off[r]) * q.stride;
+    qr.x += shift;
+    qr.y += shift;
+    qr.z += shift;
+    return qr;
+  };
+  // Pass 1: row counts of every replica; pass 2 (after the scan) rebuilds
+  // each replica's list and stores it at its place with global indices.
+  std::vector<int64_t> e(nrep, 0);
+  for (int32_t r = 0; r < nrep; r++) {
+    const int32_t nr = off[r + 1] - off[r];
+    if (nr == 0) continue;
+    LCHECK(dev_makepair(s, part_of(r), nr, boxes[r], search_length, kind, w, w.part));
+    e[r] = w.part.e;
+    LTRY(hipMemcpyAsync(out.nop.as<int32_t>() + off[r], w.part.nop.p, sizeof(int32_t) * nr,
+                        hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+  }
+  int64_t total = 0;
+  LCHECK(dev_scan(s, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), n, w, &total));
+  if (total > INT32_MAX) return set_error(LJGPU_ERR_CAPACITY, "list longer than 2^31-1 entries");
+  out.e = total;
+  LCHECK(dev_reserve(out.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  int64_t pos = 0;
+  for (int32_t r = 0; r < nrep; r++) {
+    const int32_t nr = off[r + 1] - off[r];
+    if (nr == 0) continue;
+    LCHECK(dev_makepair(s, part_of(r), nr, boxes[r], search_length, kind, w, w.part));
+    if (w.part.e != e[r]) return set_error(LJGPU_ERR_HIP, "batched list build: replica changed size");
+    if (e[r] > 0) {
+      k_copy_add<<<grid_for(e[r], kBlock), kBlock, 0, s>>>(e[r], w.part.list.as<int32_t>(),
+                                                           out.list.as<int32_t>() + pos, off[r]);
+      LTRY(hipGetLastError(), "copy kernel");
+    }
+    pos += e[r];
+  }
+  return LJGPU_OK;
+}
+
+int dev_check_replicas(hipStream_t s, const DevCSR& c, const std::vector<int32_t>& off,
+                       ListWork& w) {
+  if (c.n <= 0) return LJGPU_OK;
+  const int32_t nrep = static_cast<int32_t>(off.size()) - 1;
+  LCHECK(dev_reserve(w.rep_of, sizeof(int32_t) * c.n, s));
+  LCHECK(dev_reserve(w.tile_geo, sizeof(int32_t) * off.size(), s));
+  LTRY(hipMemcpyAsync(w.tile_geo.p, off.data(), sizeof(int32_t) * off.size(),
+                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  k_replica_of<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, w.tile_geo.as<int32_t>(), nrep,
+                                                        w.rep_of.as<int32_t>());
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  unsigned long long* bad = w.small.as<unsigned long long>();
+  LCHECK(fill<unsigned long long>(s, bad, 1, ~0ull));
+  k_check_replicas<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(
+      c.n, c.nop.as<int32_t>(), c.ptr.as<int32_t>(), c.list.as<int32_t>(), w.rep_of.as<int32_t>(),
+      bad);
+  LTRY(hipGetLastError(), "replica check kernels");
+  unsigned long long h = 0;
+  LTRY(hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (h == ~0ull) return LJGPU_OK;
+  return set_error(LJGPU_ERR_BAD_LIST,
+                   "row " + std::to_string(h) + " lists a particle of another replica");
 }
 
 }  // namespace ljgpu

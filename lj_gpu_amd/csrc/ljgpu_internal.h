@@ -59,6 +59,10 @@ struct ListView {
   const int32_t* tile_desc = nullptr;   // 8 ints per block: i0, ni, soff, ns, eoff, ne, 0, 0
   const int32_t* tile_stage = nullptr;  // staged particle indices
   const uint16_t* tile_local = nullptr; // per-entry LDS byte offsets
+
+  // Batched replicas (tiled kernel only): per replica (box, 1/box), indexed
+  // by descriptor slot 6; null for a single system.
+  const double* replica_box = nullptr;
 };
 
 // Host-built LDS tiling of a full CSR list (see ListView).

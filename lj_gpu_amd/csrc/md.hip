@@ -188,6 +188,8 @@ int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, d
                  double* potential, double* kinetic, int64_t* pairs_in_cutoff) {
   if (!ctx || !potential || !kinetic) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
   if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  if (!ctx->rep_off.empty())
+    return set_error(LJGPU_ERR_UNSUPPORTED, "energy of batched replicas");
   if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
   if (!(cl2 >= 0.0)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "cl2 must be >= 0");
   TRY(use_device(ctx));
@@ -214,6 +216,8 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: unknown accumulator");
   if (samples && sample_capacity < 0)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "md_run: negative sample capacity");
+  if (!ctx->rep_off.empty())
+    return set_error(LJGPU_ERR_UNSUPPORTED, "molecular dynamics of batched replicas");
   std::memset(stats, 0, sizeof(*stats));
   TRY(use_device(ctx));
   const double dt = cfg->dt, cl2 = cfg->cutoff * cfg->cutoff;
