@@ -55,6 +55,10 @@ extern "C" {
 #define LJGPU_MEM_HOST 0
 #define LJGPU_MEM_DEVICE 1
 
+/* Force precision: FP64 throughout, or the mixed kernels. */
+#define LJGPU_PREC_F64 0
+#define LJGPU_PREC_MIXED 1
+
 /* Per-particle accumulator of the mixed-precision kernels. */
 #define LJGPU_ACCUM_F64 0
 #define LJGPU_ACCUM_KAHAN_F32 1
@@ -207,6 +211,31 @@ int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_
                     int64_t stage_capacity, uint16_t* local, int64_t local_capacity,
                     int32_t* n_blocks, int64_t* n_stage);
 
+/* Autotuner. Times `reps` force evaluations (after as many warm-up ones)
+ * over the installed list on device AoS positions q for each candidate
+ * configuration: the tiled kernel with 4, 8 or 16 lanes per particle, unroll
+ * 2 or 4 and tiles of 16, 32 or 64 particles, one persistent grid, the
+ * untiled wavefront-segment kernel and the thread-per-particle ELL kernel.
+ * The caller's momenta are not touched (a scratch buffer is used). The
+ * fastest configuration is installed and returned in *best; the whole table
+ * stays available through ljgpu_query_autotune. No reference counterpart (the
+ * reference takes only its block size from argv, cuda/force_cuda.cu:377-383). */
+typedef struct ljgpu_tune_entry {
+  int32_t lanes_per_particle;
+  int32_t unroll;
+  int32_t tile_particles;   /* requested tile size (0: untiled kernels)   */
+  int32_t resident_per_cu;  /* persistent grid (0: one workgroup per tile) */
+  int32_t tile_used;        /* tile size the kernel ran with               */
+  float us_per_step;        /* GPU time per force evaluation               */
+} ljgpu_tune_entry;
+
+int ljgpu_autotune(ljgpu_ctx* ctx, const double* q, double box, double cl2, int precision,
+                   int accum, int32_t reps, ljgpu_tune_entry* best);
+/* The table of the last ljgpu_autotune call: *count entries, up to
+ * `capacity` copied (pass null results to query the count). */
+int ljgpu_query_autotune(ljgpu_ctx* ctx, ljgpu_tune_entry* results, int32_t capacity,
+                         int32_t* count);
+
 /* ---- forces ------------------------------------------------------------ */
 /* p += F(q) * dt over the installed list, repeated `steps` times (the
  * reference's measure() loop of 100 launches, cuda/force_cuda.cu:319-342;
@@ -248,9 +277,6 @@ int ljgpu_force_kernel_csr_f64(ljgpu_ctx* ctx, const double* q, double* p,
  * cuda/force_cuda.cu:331-335). These entry points wrap that same evaluation
  * (p += F(q) * dt, unit masses, so p is the velocity) in a velocity-Verlet
  * loop with a periodic list rebuild on the device. */
-#define LJGPU_PREC_F64 0
-#define LJGPU_PREC_MIXED 1
-
 typedef struct ljgpu_md_config {
   double dt;              /* time step                                          */
   double cutoff;          /* force cutoff: pairs with r2 > cutoff^2 are skipped  */

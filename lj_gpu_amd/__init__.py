@@ -87,6 +87,16 @@ PREC_F64 = 0
 PREC_MIXED = 1
 
 
+class TuneEntry(C.Structure):
+    """ljgpu_tune_entry (include/ljgpu.h)."""
+    _fields_ = [("lanes_per_particle", C.c_int32), ("unroll", C.c_int32),
+                ("tile_particles", C.c_int32), ("resident_per_cu", C.c_int32),
+                ("tile_used", C.c_int32), ("us_per_step", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class MDConfig(C.Structure):
     """ljgpu_md_config (include/ljgpu.h)."""
     _fields_ = [("dt", C.c_double), ("cutoff", C.c_double), ("skin", C.c_double),
@@ -152,6 +162,8 @@ def lib() -> C.CDLL:
         "ljgpu_get_tiles": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i32),
                             C.POINTER(_i64)],
         "ljgpu_set_replicas": [_vp, _i32, _vp, _vp],
+        "ljgpu_autotune": [_vp, _vp, _d, _d, C.c_int, C.c_int, _i32, C.POINTER(TuneEntry)],
+        "ljgpu_query_autotune": [_vp, C.POINTER(TuneEntry), _i32, C.POINTER(_i32)],
         "ljgpu_md_run": [_vp, _vp, _vp, _i32, _d, C.POINTER(MDConfig), _i32, C.POINTER(MDStats),
                          _vp, _i32],
         "ljgpu_energy": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
@@ -528,6 +540,20 @@ class Context:
         nrep = max(off.size - 1, 0)
         _check(lib().ljgpu_set_replicas(self.handle, nrep, _ptr(off) if nrep else None,
                                         _ptr(bx) if nrep else None))
+
+    def autotune(self, q: DeviceArray, box: float = 0.0, cl2: float = CL2,
+                 precision: int = PREC_F64, accum: int = ACCUM_F64,
+                 reps: int = 20) -> tuple[dict, list[dict]]:
+        """Time every candidate kernel configuration on the installed list,
+        install the fastest; returns (best, table)."""
+        best = TuneEntry()
+        _check(lib().ljgpu_autotune(self.handle, q.ptr, box, cl2, precision, accum, reps,
+                                    C.byref(best)))
+        cnt = _i32(0)
+        _check(lib().ljgpu_query_autotune(self.handle, None, 0, C.byref(cnt)))
+        table = (TuneEntry * max(cnt.value, 1))()
+        _check(lib().ljgpu_query_autotune(self.handle, table, cnt.value, C.byref(cnt)))
+        return best.as_dict(), [table[i].as_dict() for i in range(cnt.value)]
 
     def set_graphs(self, enable: bool) -> None:
         _check(lib().ljgpu_set_graphs(self.handle, 1 if enable else 0))

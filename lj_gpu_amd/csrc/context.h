@@ -78,6 +78,9 @@ struct ljgpu_ctx {
   std::vector<double> rep_boxes;
   ljgpu::DevMem rep_box_dev;
 
+  // Autotuner results of the last ljgpu_autotune call.
+  std::vector<ljgpu_tune_entry> tune;
+
   // Molecular-dynamics state: positions at the last list build, reduction
   // scratch.
   ljgpu::DevMem md_qref;
