@@ -20,6 +20,12 @@ Prints ONE JSON line on stdout:
                  same list
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload {large,replicas,md} [--steps K]
+The other workloads print one JSON line each in the same format: "large" the
+same evaluation on FCC 4*64^3 = 1,048,576 particles with FP64, mixed and
+Kahan-mixed kernels; "replicas" 256 periodic boxes of uneven size in one
+autotuned launch per step; "md" K velocity-Verlet steps of 8,388,608
+particles at density 0.8 with device list rebuilds every 20 steps.
 """
 from __future__ import annotations
 
@@ -88,6 +94,148 @@ def cpu_baseline(q, box, half, dt, budget_s: float) -> dict:
             "cpu": cpu_model(), "seconds": round(t, 3), "steps": steps}
 
 
+def bench_large(args, G, np) -> dict:
+    """FP64 and mixed force evaluation on FCC 4*64^3 = 1,048,576 particles
+    at density 1.0 (periodic, full list)."""
+    dt, k = 0.01, 64
+    q, box = G.lattice_periodic(k, 1.0, 2)
+    ctx = G.Context(0)
+    dq = ctx.array(q)
+    ctx.build_list(dq, q.shape[0], box, G.LIST_FULL)
+    info = ctx.list_info()
+    e, n = info.n_entries, info.particle_number
+    dp = ctx.zeros(q.shape)
+    out = {}
+    for name, fn in (("f64", lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, box, s)),
+                     ("mixed_f64acc", lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s,
+                                                                    G.ACCUM_F64)),
+                     ("mixed_kahan", lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s,
+                                                                   G.ACCUM_KAHAN_F32))):
+        fn(args.warmup)
+        ctx.sync()
+        ctx.timer_start()
+        fn(args.steps)
+        ms = ctx.timer_stop()
+        out[name] = {"ms_per_step": ms / args.steps, "gpair_s": e / 2 * args.steps / ms / 1e6}
+    errs = {}
+    dF = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dF, 1.0, G.CL2, box, 1)
+    F = dF.download()[:, :3]
+    for acc in (G.ACCUM_F64, G.ACCUM_KAHAN_F32):
+        dM = ctx.zeros(q.shape)
+        ctx.force_aos_mixed(dq, dM, 1.0, G.CL2, box, 1, acc)
+        M = dM.download()[:, :3]
+        sc = float(np.abs(F).max())
+        errs[acc] = (float(np.abs(M - F).max() / sc), float(np.sqrt(((M - F) ** 2).mean()) / sc))
+    ctx.close()
+    ms = out["f64"]["ms_per_step"]
+    return {
+        "metric": METRIC, "value": round(out["f64"]["gpair_s"], 3), "unit": "Gpair-interactions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 6),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"LJ force, FCC 4*{k}^3={n} particles, density 1.0, periodic box "
+                               f"{box:.4f}, full Verlet list built on the device, dt {dt}, AoS",
+                   "particles": n, "list_entries": e, "unique_pairs": e // 2,
+                   "lanes_per_particle": info.lanes_per_particle, "unroll": info.unroll,
+                   "tile_particles": info.tile_particles},
+        "roofline": roofline(4 * e + 104 * n, ms / 1e3),
+        "mixed": {"f64acc": {"value": round(out["mixed_f64acc"]["gpair_s"], 3),
+                             "speedup_vs_f64": round(ms / out["mixed_f64acc"]["ms_per_step"], 4),
+                             "max_rel_force_err_vs_f64": errs[G.ACCUM_F64][0],
+                             "rms_rel_force_err_vs_f64": errs[G.ACCUM_F64][1]},
+                  "kahan": {"value": round(out["mixed_kahan"]["gpair_s"], 3),
+                            "speedup_vs_f64": round(ms / out["mixed_kahan"]["ms_per_step"], 4),
+                            "max_rel_force_err_vs_f64": errs[G.ACCUM_KAHAN_F32][0],
+                            "rms_rel_force_err_vs_f64": errs[G.ACCUM_KAHAN_F32][1]}},
+        "cpu_baseline": None,
+    }
+
+
+def bench_replicas(args, G, np) -> dict:
+    """256 periodic replica boxes (FCC 4*k^3, k uniform in 6..20, density
+    uniform in [0.5, 1.0]) in one tiled launch per step, configuration from
+    the autotuner, 100-step loops replayed as HIP graphs."""
+    dt = 0.01
+    q, off, boxes = G.replica_workload(256, seed=4)
+    ctx = G.Context(0)
+    ctx.set_replicas(off, boxes)
+    dq = ctx.array(q)
+    ctx.build_list(dq, q.shape[0], 0.0, G.LIST_FULL)
+    best, table = ctx.autotune(dq, 0.0, reps=20)
+    info = ctx.list_info()
+    e, n = info.n_entries, info.particle_number
+    dp = ctx.zeros(q.shape)
+    res = {}
+    for name, fn in (("f64", lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, 0.0, s)),
+                     ("mixed", lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, 0.0, s,
+                                                             G.ACCUM_F64))):
+        fn(args.warmup)
+        ctx.sync()
+        ctx.timer_start()
+        fn(args.steps)
+        res[name] = ctx.timer_stop() / args.steps
+    ctx.close()
+    ms = res["f64"]
+    return {
+        "metric": METRIC, "value": round(e / 2 / (ms / 1e3) / 1e9, 3),
+        "unit": "Gpair-interactions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"256 periodic LJ replicas, FCC 4*k^3 with k in 6..20, density in "
+                               f"[0.5, 1], {n} particles in total, one tiled launch per step, "
+                               f"dt {dt}, AoS", "particles": n, "replicas": 256,
+                   "list_entries": e, "unique_pairs": e // 2, "autotuned": best},
+        "roofline": roofline(4 * e + 104 * n, ms / 1e3),
+        "mixed": {"value": round(e / 2 / (res["mixed"] / 1e3) / 1e9, 3),
+                  "speedup_vs_f64": round(ms / res["mixed"], 4)},
+        "autotune_table": table,
+        "cpu_baseline": None,
+    }
+
+
+def bench_md(args, G, np) -> dict:
+    """Velocity Verlet on FCC 4*128^3 = 8,388,608 particles at density 0.8,
+    Maxwell-Boltzmann momenta at T = 1 (seed 7), dt 0.005, list rebuilt on
+    the device every 20 steps with skin 0.3, mixed-precision forces."""
+    q, box = G.lattice_periodic(128, 0.8, 2)
+    p = G.maxwell_boltzmann(q.shape[0], 1.0, 7)
+    ctx = G.Context(0)
+    dq, dp = ctx.array(q), ctx.array(p)
+    steps = args.steps
+    st, samples = ctx.md_run(dq, dp, box, steps, 0.005, skin=0.3, rebuild_every=20,
+                             precision=G.PREC_MIXED, sample_every=max(steps // 10, 1))
+    ctx.close()
+    n = q.shape[0]
+    pairs = st.pair_entries / 2
+    total_s = st.ms_total / 1e3
+    return {
+        "metric": METRIC, "value": round(pairs / total_s / 1e9, 3),
+        "unit": "Gpair-interactions/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(st.ms_total / max(steps, 1), 6), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64-mixed", "data": "synthetic",
+        "config": {"workload": f"LJ velocity Verlet, FCC 4*128^3={n} particles, density 0.8, "
+                               f"periodic box {box:.4f}, T=1 Maxwell-Boltzmann (seed 7), dt 0.005, "
+                               "device list rebuild every 20 steps with skin 0.3, mixed forces, "
+                               "FP64 integration", "particles": n},
+        "md": {"rebuilds": st.rebuilds, "ms_rebuild": st.ms_rebuild, "ms_total": st.ms_total,
+               "stale_lists": st.stale_lists, "max_displacement": st.max_displacement,
+               "energy_start": st.potential0 + st.kinetic0,
+               "energy_end": st.potential1 + st.kinetic1,
+               "max_abs_energy_drift": st.max_abs_drift,
+               "relative_energy_drift": st.max_abs_drift / abs(st.potential0 + st.kinetic0),
+               "samples": samples.tolist()},
+        "cpu_baseline": None,
+    }
+
+
+def roofline(bytes_per_launch: int, seconds: float, traffic=None) -> dict:
+    achieved = bytes_per_launch / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(seconds * 1e6, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +250,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
+    ap.add_argument("--workload", choices=["force", "large", "replicas", "md"], default="force",
+                    help="force: the headline N=119,164 force evaluation (default); large: "
+                         "N=1,048,576; replicas: 256 batched boxes; md: 8.4M-particle velocity "
+                         "Verlet with device list rebuilds")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after timing, run three device-to-device copies of this many MiB "
                          "(a known-size kernel that calibrates rocprofv3's byte counters)")
@@ -117,6 +269,11 @@ def main() -> int:
     import numpy as np
 
     import lj_gpu_amd as G
+
+    if args.workload != "force":
+        fn = {"large": bench_large, "replicas": bench_replicas, "md": bench_md}[args.workload]
+        print(json.dumps(fn(args, G, np)), flush=True)
+        return 0
 
     dt = 0.01
     t0 = time.time()

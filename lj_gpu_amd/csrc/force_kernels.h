@@ -220,6 +220,15 @@ __device__ __forceinline__ void unwrap_near(double& x, double& y, double& z, dou
   }
 }
 
+// Loads from the workgroup's LDS at byte address `a`. The tile kernel uses
+// dynamic LDS only, which starts at address 0, so an offset into it is its
+// LDS address; building the pointer from the integer lets the compiler fold
+// constant displacements into the DS instruction's offset field.
+template <class T>
+__device__ __forceinline__ T lds_at(uint32_t a) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) T*>(static_cast<uintptr_t>(a));
+}
+
 // One row of an LDS tile: entries [0, np) of the row, lane `sub` of the
 // segment taking entries base + u*TPI + sub. The row's entry offsets
 // (`lrow`) and the partner positions (`lb`) are both in LDS. Full batches of
@@ -228,92 +237,96 @@ __device__ __forceinline__ void unwrap_near(double& x, double& y, double& z, dou
 // entries). MI applies the minimum image per entry (blocks whose unwrapped
 // frame is not exact).
 template <int TPI, int U, int PREC, bool MI, class M>
-__device__ __forceinline__ void tile_row(const M& mom, const char* lb, const uint16_t* lrow,
-                                         int np, int sub, double xi, double yi, double zi,
-                                         const PairParams& pp, Accum<PREC>& acc) {
+__device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int np, int sub,
+                                         double xi, double yi, double zi, const PairParams& pp,
+                                         Accum<PREC>& acc) {
   constexpr int kB = U * TPI;
   int base = 0;
   for (; base + kB <= np; base += kB) {
     uint32_t off[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) off[u] = lrow[base + u * TPI + sub];
+    for (int u = 0; u < U; u++) off[u] = kTileLdsHeader + lrow[base + u * TPI + sub];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const double xj = *reinterpret_cast<const double*>(lb + off[u]);
-      const double yj = *reinterpret_cast<const double*>(lb + off[u] + 8);
-      const double zj = *reinterpret_cast<const double*>(lb + off[u] + 16);
+      const double xj = lds_at<double>(off[u]);
+      const double yj = lds_at<double>(off[u] + 8);
+      const double zj = lds_at<double>(off[u] + 16);
       pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
     }
   }
   for (int k = base + sub; k < np; k += TPI) {
-    const uint32_t o = lrow[k];
-    const double xj = *reinterpret_cast<const double*>(lb + o);
-    const double yj = *reinterpret_cast<const double*>(lb + o + 8);
-    const double zj = *reinterpret_cast<const double*>(lb + o + 16);
+    const uint32_t o = kTileLdsHeader + lrow[k];
+    const double xj = lds_at<double>(o);
+    const double yj = lds_at<double>(o + 8);
+    const double zj = lds_at<double>(o + 16);
     pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
   }
 }
 
-// One full batch of a tile row: all U partner positions are read from LDS
-// before any is used, then the U entries are computed; the offsets buffer
-// `o` is refilled with the batch two ahead (`refill` >= 0) in between, so
-// the global offset loads have two batches of arithmetic to land.
-template <int TPI, int U, int PREC, bool MI, class M>
-__device__ __forceinline__ void tile_batch(const M& mom, const char* lb,
-                                           const uint16_t* __restrict__ lrow, uint32_t (&o)[U],
-                                           int refill, int sub, double xi, double yi, double zi,
+// One batch of a tile row: all U partner positions are read from LDS before
+// any is used, then the U entries are computed; the offsets buffer `o` is
+// refilled in between with the batch two ahead (loads from global memory
+// that then have two batches of arithmetic to land). The offset array is
+// padded so these loads stay inside it past the end of the last row; what
+// they read beyond a row is never used. LIMIT masks entries at or beyond
+// `np` (the last, partial batch of a row): their offsets are replaced by 0
+// (a valid slot) and their force by 0.
+template <int TPI, int U, int PREC, bool MI, bool LIMIT, bool REFILL, class M>
+__device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restrict__ lrow,
+                                           uint32_t (&o)[U], int base, int refill, int np,
+                                           int sub, double xi, double yi, double zi,
                                            const PairParams& pp, Accum<PREC>& acc) {
   double xj[U], yj[U], zj[U];
+  bool valid[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    xj[u] = *reinterpret_cast<const double*>(lb + o[u]);
-    yj[u] = *reinterpret_cast<const double*>(lb + o[u] + 8);
-    zj[u] = *reinterpret_cast<const double*>(lb + o[u] + 16);
+    valid[u] = !LIMIT || base + u * TPI + sub < np;
+    const uint32_t a = kTileLdsHeader + (valid[u] ? o[u] : 0u);
+    xj[u] = lds_at<double>(a);
+    yj[u] = lds_at<double>(a + 8);
+    zj[u] = lds_at<double>(a + 16);
   }
-  if (refill >= 0) {
+  if constexpr (REFILL) {
 #pragma unroll
     for (int u = 0; u < U; u++) o[u] = lrow[refill + u * TPI + sub];
   }
 #pragma unroll
   for (int u = 0; u < U; u++)
-    pair_compute<PREC, MI, false>(mom, 0, true, xj[u] - xi, yj[u] - yi, zj[u] - zi, pp, acc);
+    pair_compute<PREC, MI, false>(mom, 0, valid[u], xj[u] - xi, yj[u] - yi, zj[u] - zi, pp, acc);
 }
 
 // As tile_row, with the row's entry offsets read from global memory
 // (`lrow` in HBM) through two alternating offset buffers, each refilled two
-// batches ahead.
+// batches ahead, and the row's last partial batch run as one masked batch.
 template <int TPI, int U, int PREC, bool MI, class M>
-__device__ __forceinline__ void tile_row_stream(const M& mom, const char* lb,
-                                                const uint16_t* __restrict__ lrow, int np,
-                                                int sub, double xi, double yi, double zi,
+__device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __restrict__ lrow,
+                                                int np, int sub, double xi, double yi, double zi,
                                                 const PairParams& pp, Accum<PREC>& acc) {
   constexpr int kB = U * TPI;
+  const int nfull = np / kB;
   uint32_t oa[U], ob[U];
-  if (kB <= np) {
 #pragma unroll
-    for (int u = 0; u < U; u++) oa[u] = lrow[u * TPI + sub];
+  for (int u = 0; u < U; u++) {
+    oa[u] = lrow[u * TPI + sub];
+    ob[u] = lrow[kB + u * TPI + sub];
   }
-  if (2 * kB <= np) {
+  int b = 0;
+  for (; b + 2 <= nfull; b += 2) {
+    tile_batch<TPI, U, PREC, MI, false, true>(mom, lrow, oa, b * kB, (b + 2) * kB, np, sub, xi,
+                                              yi, zi, pp, acc);
+    tile_batch<TPI, U, PREC, MI, false, true>(mom, lrow, ob, (b + 1) * kB, (b + 3) * kB, np, sub,
+                                              xi, yi, zi, pp, acc);
+  }
+  if (b < nfull) {
+    tile_batch<TPI, U, PREC, MI, false, false>(mom, lrow, oa, b * kB, 0, np, sub, xi, yi, zi, pp,
+                                               acc);
+    b++;
 #pragma unroll
-    for (int u = 0; u < U; u++) ob[u] = lrow[kB + u * TPI + sub];
+    for (int u = 0; u < U; u++) oa[u] = ob[u];
   }
-  int base = 0;
-  while (base + kB <= np) {
-    tile_batch<TPI, U, PREC, MI>(mom, lb, lrow, oa, base + 3 * kB <= np ? base + 2 * kB : -1,
-                                 sub, xi, yi, zi, pp, acc);
-    base += kB;
-    if (base + kB > np) break;
-    tile_batch<TPI, U, PREC, MI>(mom, lb, lrow, ob, base + 3 * kB <= np ? base + 2 * kB : -1,
-                                 sub, xi, yi, zi, pp, acc);
-    base += kB;
-  }
-  for (int k = base + sub; k < np; k += TPI) {
-    const uint32_t o = lrow[k];
-    const double xj = *reinterpret_cast<const double*>(lb + o);
-    const double yj = *reinterpret_cast<const double*>(lb + o + 8);
-    const double zj = *reinterpret_cast<const double*>(lb + o + 16);
-    pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
-  }
+  if (b * kB < np)
+    tile_batch<TPI, U, PREC, MI, true, false>(mom, lrow, oa, b * kB, 0, np, sub, xi, yi, zi, pp,
+                                              acc);
 }
 
 // LDS-tiled full-list kernel: one workgroup per block of consecutive
@@ -434,15 +447,15 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   if (active) {
     if constexpr (OFFS_LDS) {
       if (PBC && any_far)
-        tile_row<TPI, U, PREC, true>(mom, lb, loff + rs, np, sub, xi, yi, zi, pp, acc);
+        tile_row<TPI, U, PREC, true>(mom, loff + rs, np, sub, xi, yi, zi, pp, acc);
       else
-        tile_row<TPI, U, PREC, false>(mom, lb, loff + rs, np, sub, xi, yi, zi, pp, acc);
+        tile_row<TPI, U, PREC, false>(mom, loff + rs, np, sub, xi, yi, zi, pp, acc);
     } else {
       const uint16_t* grow = local + eoff + rs;
       if (PBC && any_far)
-        tile_row_stream<TPI, U, PREC, true>(mom, lb, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, true>(mom, grow, np, sub, xi, yi, zi, pp, acc);
       else
-        tile_row_stream<TPI, U, PREC, false>(mom, lb, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, false>(mom, grow, np, sub, xi, yi, zi, pp, acc);
     }
   }
   const double fx = segment_sum<TPI>(acc.vx());

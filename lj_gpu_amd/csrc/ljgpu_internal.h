@@ -78,11 +78,13 @@ struct HostTiles {
 // LDS budget of one tile: staged positions (24 bytes each, addressed by
 // 16-bit byte offsets), then the tile's 16-bit entry offsets, copied in
 // 16-byte chunks from the 8-entry boundary below the first entry (up to 14
-// extra entries). The device copy of the offsets is padded by
-// kTileLocalPad entries so the last chunk stays inside the allocation.
+// extra entries). The device copy of the offsets is padded by kTileLocalPad
+// zero entries: the last chunk copy and the kernel's offset prefetch, which
+// reads up to two batches (2 * 4 * 64 entries) past the end of a row, stay
+// inside the allocation.
 constexpr int32_t kTileLdsHeader = 16;  // kernel's own word(s) ahead of the tile data
 constexpr int32_t kTileLdsBudget = 65536 - kTileLdsHeader;
-constexpr int32_t kTileLocalPad = 8;
+constexpr int32_t kTileLocalPad = 512;
 LJ_HD inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
   return ((ns * 24 + 15) / 16) * 16 + ((ne + 14 + 7) / 8) * 16;
 }
