@@ -254,6 +254,9 @@ def main() -> int:
                     help="force: the headline N=119,164 force evaluation (default); large: "
                          "N=1,048,576; replicas: 256 batched boxes; md: 8.4M-particle velocity "
                          "Verlet with device list rebuilds")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every step individually instead of replaying captured graphs "
+                         "(for per-dispatch profiler counters)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after timing, run three device-to-device copies of this many MiB "
                          "(a known-size kernel that calibrates rocprofv3's byte counters)")
@@ -285,6 +288,8 @@ def main() -> int:
     log(f"setup: N={n} box={box:.4f} full entries={e} pairs={pairs} ({time.time() - t0:.1f}s)")
 
     ctx = G.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     ctx.set_kernel_config(args.lanes, args.block, args.unroll)
     if args.tile >= 0:
         ctx.set_tiling(args.tile)

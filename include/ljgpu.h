@@ -185,7 +185,7 @@ int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode);
  * the block size (argv[1], cuda/force_cuda.cu:377-383). */
 int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_size, int unroll);
 
-/* LDS tiling of full lists (on by default, 32 particles per tile): the list
+/* LDS tiling of full lists (on by default, 64 particles per tile): the list
  * is regrouped once per ljgpu_set_list_* into blocks of consecutive
  * particles whose partners' positions are staged in LDS, and each entry is
  * re-expressed as a 16-bit LDS offset. 0 turns tiling off (global-gather

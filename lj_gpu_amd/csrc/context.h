@@ -57,7 +57,7 @@ struct ljgpu_ctx {
   bool ell_ready = false;
 
   // LDS tiling of the evaluated full list (tile_particles > 0 enables it).
-  int32_t tile_particles = 32;
+  int32_t tile_particles = 64;
   bool tile_offsets_lds = false;  // stage entry offsets in LDS (else stream them)
   int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
   int cus = 0;                    // compute units of the device
@@ -86,7 +86,7 @@ struct ljgpu_ctx {
   ljgpu::DevMem md_qref;
   ljgpu::DevMem md_red;
 
-  int lanes = 8;
+  int lanes = 4;
   int unroll = 4;
   int block = 256;
 };

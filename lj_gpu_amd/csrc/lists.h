@@ -66,6 +66,7 @@ struct ListWork {
   DevMem tile_geo;    // int32 x 4 per tile: first particle, count, replica
   DevMem rep_of;      // int32 per particle: replica index
   DevCSR part;        // one replica's list during a batched build
+  int32_t last_max_row = 0;  // longest row of the last device build (slab size hint)
 };
 void dev_release(ListWork& w);
 

@@ -251,44 +251,49 @@ __device__ __forceinline__ void sort_segment(const int32_t* __restrict__ src,
   }
 }
 
+// Row r is read from src + r * src_stride (src_stride > 0, a slab) or from
+// src + start[r], and written to dst + start[r].
 __global__ __launch_bounds__(kBlock) void k_row_sort(int32_t n, const int32_t* __restrict__ len,
                                                      const int32_t* __restrict__ start,
                                                      const int32_t* __restrict__ src,
+                                                     int64_t src_stride,
                                                      int32_t* __restrict__ dst) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
   for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6); r < n;
        r += waves) {
     const int32_t m = len[r];
-    const int64_t s = start[r];
+    const int64_t d = start[r];
+    const int32_t* in = src + (src_stride > 0 ? r * src_stride : d);
+    int32_t* o = dst + d;
     if (m <= 1) {
-      if (m == 1 && lane == 0) dst[s] = src[s];
+      if (m == 1 && lane == 0) o[0] = in[0];
     } else if (m <= 64) {
-      sort_segment<1>(src + s, dst + s, m, lane);
+      sort_segment<1>(in, o, m, lane);
     } else if (m <= 128) {
-      sort_segment<2>(src + s, dst + s, m, lane);
+      sort_segment<2>(in, o, m, lane);
     } else if (m <= 256) {
-      sort_segment<4>(src + s, dst + s, m, lane);
+      sort_segment<4>(in, o, m, lane);
     } else if (m <= 512) {
-      sort_segment<8>(src + s, dst + s, m, lane);
+      sort_segment<8>(in, o, m, lane);
     } else {
       for (int32_t a = lane; a < m; a += 64) {
-        const int32_t x = src[s + a];
+        const int32_t x = in[a];
         int32_t rank = 0;
         for (int32_t b = 0; b < m; b++) {
-          const int32_t y = src[s + b];
+          const int32_t y = in[b];
           rank += (y < x) | ((y == x) & (b < a));
         }
-        dst[s + rank] = x;
+        o[rank] = x;
       }
     }
   }
 }
 
 int row_sort(hipStream_t s, int32_t n, const int32_t* len, const int32_t* start,
-             const int32_t* src, int32_t* dst) {
+             const int32_t* src, int32_t* dst, int64_t src_stride = 0) {
   if (n <= 0) return LJGPU_OK;
-  k_row_sort<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(n, len, start, src, dst);
+  k_row_sort<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(n, len, start, src, src_stride, dst);
   LTRY(hipGetLastError(), "row sort kernel");
   return LJGPU_OK;
 }
@@ -468,14 +473,17 @@ __device__ __forceinline__ bool within_search(const double4& a, const double4& b
 }
 
 // One thread per particle in cell order (lanes of a wavefront share their
-// home cell, so candidate loads are mostly broadcasts). FILL = false counts
-// each row's partners into nop[i]; FILL = true writes them to
-// out[ptr[i] ..) in candidate order.
-template <bool FILL, bool HALF>
+// home cell, so candidate loads are mostly broadcasts), over the cells within
+// R of its own (R = 1 for cells at least the search length wide, 2 for cells
+// at least half of it; R = 0: one cell holding every particle). MODE 0 counts
+// each row's partners into nop[i]; MODE 1 writes them to out[ptr[i] ..) in
+// candidate order; MODE 2 does both, writing at most `cap` of them to the
+// row's slab out[i * cap ..).
+template <int MODE, bool HALF, int R>
 __global__ __launch_bounds__(kBlock) void k_pairs(int32_t n, const double4* __restrict__ cpos,
                                                   const int32_t* __restrict__ cell_sorted,
                                                   const int32_t* __restrict__ cell_start,
-                                                  GridArgs g, int stencil, double sl2,
+                                                  GridArgs g, double sl2, int32_t cap,
                                                   int32_t* __restrict__ nop,
                                                   const int32_t* __restrict__ ptr,
                                                   int32_t* __restrict__ out) {
@@ -487,37 +495,46 @@ __global__ __launch_bounds__(kBlock) void k_pairs(int32_t n, const double4* __re
   const int cx = c % g.nc0, cy = (c / g.nc0) % g.nc1, cz = c / (g.nc0 * g.nc1);
   const bool periodic = g.box > 0.0;
   const double half_box = 0.5 * g.box;
-  const int lo = stencil == 3 ? -1 : 0, hi = stencil == 3 ? 1 : 0;
   int64_t base = 0;
-  if (FILL) base = ptr[i];
+  if (MODE == 1) base = ptr[i];
+  if (MODE == 2) base = static_cast<int64_t>(i) * cap;
   int32_t cnt = 0;
-  for (int dz = lo; dz <= hi; dz++) {
+  for (int dz = -R; dz <= R; dz++) {
     int z = cz + dz;
     if (periodic) z = (z + g.nc2) % g.nc2;
     else if (z < 0 || z >= g.nc2) continue;
-    for (int dy = lo; dy <= hi; dy++) {
+    for (int dy = -R; dy <= R; dy++) {
       int y = cy + dy;
       if (periodic) y = (y + g.nc1) % g.nc1;
       else if (y < 0 || y >= g.nc1) continue;
-      for (int dx = lo; dx <= hi; dx++) {
+      for (int dx = -R; dx <= R; dx++) {
         int x = cx + dx;
         if (periodic) x = (x + g.nc0) % g.nc0;
         else if (x < 0 || x >= g.nc0) continue;
         const int32_t cc = (z * g.nc1 + y) * g.nc0 + x;
         const int32_t u1 = cell_start[cc + 1];
-        for (int32_t u = cell_start[cc]; u < u1; u++) {
-          const double4 pj = cpos[u];
+        auto visit = [&](const double4& pj) {
           const int32_t j = static_cast<int32_t>(pj.w);
-          if (HALF ? (j <= i) : (j == i)) continue;
+          if (HALF ? (j <= i) : (j == i)) return;
           if (within_search(pi, pj, g.box, half_box, sl2)) {
-            if (FILL) out[base + cnt] = j;
+            if (MODE == 1 || (MODE == 2 && cnt < cap)) out[base + cnt] = j;
             cnt++;
           }
+        };
+        int32_t u = cell_start[cc];
+        // Four candidates loaded before any is tested, so their loads overlap.
+        for (; u + 4 <= u1; u += 4) {
+          const double4 a = cpos[u], b = cpos[u + 1], c4 = cpos[u + 2], d = cpos[u + 3];
+          visit(a);
+          visit(b);
+          visit(c4);
+          visit(d);
         }
+        for (; u < u1; u++) visit(cpos[u]);
       }
     }
   }
-  if (!FILL) nop[i] = cnt;
+  if (MODE != 1) nop[i] = cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -904,8 +921,16 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
         return set_error(LJGPU_ERR_INVALID_ARGUMENT, "positions are not finite");
     }
   }
+  // Cells at least half the search length wide with a radius-2 stencil
+  // visit about 45 % fewer candidates than search-length cells with a
+  // radius-1 stencil; a periodic box needs 5 such cells per side (each cell
+  // visited once), else the coarser grid is used.
   CellGrid cg;
-  make_cell_grid(n, lo, hi, box, search_length, &cg);
+  make_cell_grid(n, lo, hi, box, 0.5 * search_length, &cg);
+  bool fine = !cg.brute;
+  for (int c = 0; c < 3; c++)
+    if (box > 0.0 && cg.nc[c] < 5) fine = false;
+  if (!fine) make_cell_grid(n, lo, hi, box, search_length, &cg);
   GridArgs g{cg.nc[0], cg.nc[1], cg.nc[2], cg.origin[0], cg.origin[1], cg.origin[2],
              cg.width[0], cg.width[1], cg.width[2], box};
   const int64_t ncell = cg.ncell;
@@ -928,26 +953,66 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
                                                         w.cpos.as<double4>(),
                                                         w.cell_sorted.as<int32_t>());
   LTRY(hipGetLastError(), "cell scatter kernel");
-  const int stencil = cg.brute ? 1 : 3;
   const bool half = kind == LJGPU_LIST_HALF;
+  const int R = cg.brute ? 0 : (fine ? 2 : 1);
   const int grid = grid_for(n, kBlock, INT32_MAX);
-  auto pairs = [&](auto fill_tag, int32_t* nop, const int32_t* ptr, int32_t* outl) {
-    constexpr bool F = decltype(fill_tag)::value;
-    if (half)
-      k_pairs<F, true><<<grid, kBlock, 0, s>>>(n, w.cpos.as<double4>(), w.cell_sorted.as<int32_t>(),
-                                               cstart, g, stencil, sl2, nop, ptr, outl);
-    else
-      k_pairs<F, false><<<grid, kBlock, 0, s>>>(n, w.cpos.as<double4>(), w.cell_sorted.as<int32_t>(),
-                                                cstart, g, stencil, sl2, nop, ptr, outl);
+  auto pairs = [&](auto mode_tag, int32_t cap, int32_t* nop, const int32_t* ptr, int32_t* outl) {
+    constexpr int M = decltype(mode_tag)::value;
+    const double4* cp = w.cpos.as<double4>();
+    const int32_t* cs = w.cell_sorted.as<int32_t>();
+#define LJ_PAIRS(H, RR) \
+  k_pairs<M, H, RR><<<grid, kBlock, 0, s>>>(n, cp, cs, cstart, g, sl2, cap, nop, ptr, outl)
+    if (half) {
+      if (R == 0) LJ_PAIRS(true, 0);
+      else if (R == 1) LJ_PAIRS(true, 1);
+      else LJ_PAIRS(true, 2);
+    } else {
+      if (R == 0) LJ_PAIRS(false, 0);
+      else if (R == 1) LJ_PAIRS(false, 1);
+      else LJ_PAIRS(false, 2);
+    }
+#undef LJ_PAIRS
   };
-  pairs(std::false_type{}, out.nop.as<int32_t>(), nullptr, nullptr);
-  LTRY(hipGetLastError(), "pair count kernel");
+  // Single pass into a slab of `cap` entries per row, sized from the last
+  // build's longest row (or from the mean density), then one sort from the
+  // slab into place; a row longer than the slab falls back to a second,
+  // exact-size pass. Without room for a slab: count pass, then fill pass.
+  int32_t cap = 0;
+  {
+    double vol = 1.0;
+    for (int c = 0; c < 3; c++) vol *= std::max(cg.extent[c], search_length);
+    const double expect = 4.18879020478639 * search_length * search_length * search_length *
+                          (static_cast<double>(n) / vol) * (half ? 0.5 : 1.0);
+    const double want = w.last_max_row > 0 ? 1.2 * w.last_max_row + 16 : 1.5 * expect + 32;
+    cap = static_cast<int32_t>(std::min(4096.0, std::max(64.0, want)));
+    cap = (cap + 31) / 32 * 32;
+    if (static_cast<double>(n) * cap * 4.0 > 16.0 * (1u << 30)) cap = 0;
+  }
+  int32_t mx = 0;
+  if (cap > 0) {
+    LCHECK(dev_reserve(w.tmp, sizeof(int32_t) * static_cast<size_t>(n) * cap, s));
+    pairs(std::integral_constant<int, 2>{}, cap, out.nop.as<int32_t>(), nullptr,
+          w.tmp.as<int32_t>());
+    LTRY(hipGetLastError(), "pair kernel");
+    int32_t mn = 0;
+    LCHECK(dev_minmax(s, out.nop.as<int32_t>(), n, w, &mn, &mx));
+  } else {
+    pairs(std::integral_constant<int, 0>{}, 0, out.nop.as<int32_t>(), nullptr, nullptr);
+    LTRY(hipGetLastError(), "pair count kernel");
+  }
   LCHECK(dev_scan(s, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), n, w, &total));
   if (total > INT32_MAX) return set_error(LJGPU_ERR_CAPACITY, "list longer than 2^31-1 entries");
   out.e = total;
   LCHECK(dev_reserve(out.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
+  if (cap > 0 && mx <= cap) {
+    w.last_max_row = mx;
+    return row_sort(s, n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(),
+                    out.list.as<int32_t>(), cap);
+  }
+  if (cap > 0) w.last_max_row = mx;
   LCHECK(dev_reserve(w.tmp, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
-  pairs(std::true_type{}, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>());
+  pairs(std::integral_constant<int, 1>{}, 0, out.nop.as<int32_t>(), out.ptr.as<int32_t>(),
+        w.tmp.as<int32_t>());
   LTRY(hipGetLastError(), "pair fill kernel");
   return row_sort(s, n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(),
                   out.list.as<int32_t>());

@@ -89,8 +89,11 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 
 // Potential energy: sum over the list of 4 (r^-12 - r^-6) for r2 <= cl2
 // (the force kernels' cutoff rule), and the number of those pairs, times
-// `weight` (1/2 for a full list, which holds every pair twice). One partial
-// sum of each per workgroup.
+// `weight` (1/2 for a full list, which holds every pair twice). Eight lanes
+// share a row (lane k takes entries k, k + 8, ...), so index loads are
+// contiguous; one partial sum of each quantity per workgroup.
+constexpr int kEnergyLanes = 8;
+
 __global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* __restrict__ nop,
                                                       const int32_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ list,
@@ -100,12 +103,15 @@ __global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* 
                                                       double* __restrict__ partial_count) {
   __shared__ double sh[kBlock / 64];
   double u = 0.0, cnt = 0.0;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  const int sub = static_cast<int>(threadIdx.x) % kEnergyLanes;
+  const int64_t rows_per_pass = static_cast<int64_t>(gridDim.x) * (kBlock / kEnergyLanes);
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(kBlock / kEnergyLanes) +
+                   threadIdx.x / kEnergyLanes;
+       i < n; i += rows_per_pass) {
     const double xi = q[4 * i], yi = q[4 * i + 1], zi = q[4 * i + 2];
     const int64_t kp = ptr[i];
-    double ui = 0.0;
-    for (int32_t k = 0; k < nop[i]; k++) {
+    const int32_t np = nop[i];
+    for (int32_t k = sub; k < np; k += kEnergyLanes) {
       const int64_t j = list[kp + k];
       const double dx = image(q[4 * j] - xi, box);
       const double dy = image(q[4 * j + 1] - yi, box);
@@ -114,10 +120,9 @@ __global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* 
       if (r2 > cl2) continue;
       const double s = 1.0 / r2;
       const double s3 = s * s * s;
-      ui += 4.0 * s3 * (s3 - 1.0);
+      u += 4.0 * s3 * (s3 - 1.0);
       cnt += 1.0;
     }
-    u += ui;
   }
   const double t = block_sum(u * weight, sh);
   const double c = block_sum(cnt * weight, sh);
