@@ -90,6 +90,9 @@ int prepare_layouts(ljgpu_ctx* ctx) {
 // the evaluated list and its layouts.
 int derive_list(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* ptr,
                 const int32_t* list, int64_t e, int kind) {
+  if (!ctx->rep_off.empty() && n != ctx->rep_off.back())
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT,
+                     "particle_number differs from the replicas' total");
   ctx->n = n;
   ctx->kind = kind;
   ctx->n_entries = e;

@@ -135,6 +135,11 @@ def test_batched_rejections(ctx, work):
     with pytest.raises(G.LJGPUError) as e:
         ctx.set_list(G.CSR(bad.number_of_partners, bad.pointer, lst, G.LIST_FULL))
     assert e.value.code == G.ERR_BAD_LIST
+    # A list over fewer particles than the replicas hold.
+    last = host_batch_list(q[:off[-2]], off[:-1], boxes[:-1], G.LIST_FULL)
+    with pytest.raises(G.LJGPUError) as e:
+        ctx.set_list(last)
+    assert e.value.code == G.ERR_INVALID_ARGUMENT
     dq = ctx.array(q)
     with pytest.raises(G.LJGPUError):
         ctx.build_list(dq, q.shape[0], boxes[0], G.LIST_FULL)  # box must be 0
