@@ -28,6 +28,8 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libljgpu.so")
+# The benchmark program (harness/force_ljgpu.cpp), built beside the library.
+HARNESS_PATH = os.path.join(PKG, "force_ljgpu")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "ljgpu.h")
 
 # Status codes and enums (include/ljgpu.h).

@@ -6,6 +6,8 @@ HIP sources are compiled by hipcc for gfx950 with packed FP32 instructions
 disabled; host.cpp by g++ with -ffp-contract=off (the neighbour-list
 predicate must round exactly as the reference's makepair); the objects are
 linked by hipcc into one shared library with a C ABI (include/ljgpu.h).
+The benchmark program lj_gpu_amd/force_ljgpu (harness/force_ljgpu.cpp, the
+reference's cuda/force_cuda.cu main over that ABI) is linked against it.
 """
 from __future__ import annotations
 
@@ -20,6 +22,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libljgpu.so")
+HARNESS = os.path.join(PKG, "force_ljgpu")
 
 ARCH = os.environ.get("LJGPU_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -37,6 +40,9 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall"]
+# The benchmark program over the C ABI (harness/force_ljgpu.cpp): a plain
+# g++ link against the library, found next to the executable at run time.
+HARNESS_FLAGS = ["-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -73,6 +79,12 @@ def build(verbose: bool = False) -> str:
         print(" ".join(link))
     _run(link)
     os.replace(tmp, LIB)
+    exe = [CXX, *HARNESS_FLAGS, os.path.join(PKG, "harness", "force_ljgpu.cpp"), "-o",
+           HARNESS + ".tmp", f"-L{PKG}", "-lljgpu", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(exe))
+    _run(exe)
+    os.replace(HARNESS + ".tmp", HARNESS)
     return LIB
 
 
