@@ -153,7 +153,8 @@ int install_host(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* p
 }
 
 // Build the list from device positions (cell list on the device).
-int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind) {
+int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind,
+               bool sort_rows) {
   if (!(search > 0.0) || n < 0)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "build_list: bad argument");
   if (kind != LJGPU_LIST_HALF && kind != LJGPU_LIST_FULL)
@@ -183,7 +184,8 @@ int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double sea
   // directly as the full list.
   const int build_kind = expand ? LJGPU_LIST_FULL : LJGPU_LIST_HALF;
   if (ctx->rep_off.empty())
-    TRY(dev_makepair(ctx->stream, q, n, box, search, build_kind, ctx->work, ctx->eval));
+    TRY(dev_makepair(ctx->stream, q, n, box, search, build_kind, ctx->work, ctx->eval,
+                     sort_rows));
   else
     TRY(dev_makepair_batch(ctx->stream, q, ctx->rep_off, ctx->rep_boxes, search, build_kind,
                            ctx->work, ctx->eval));

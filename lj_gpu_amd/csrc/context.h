@@ -113,7 +113,8 @@ inline int hip_fail(hipError_t e, const char* what) {
 int use_device(ljgpu_ctx* ctx);
 int prepare_layouts(ljgpu_ctx* ctx);
 // Build the list from device positions and install it (ljgpu_build_list).
-int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind);
+int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind,
+               bool sort_rows = true);
 // Enqueue one force evaluation p += F(q) * dt over the installed list.
 int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum);
 // `steps` force evaluations (graph replays where possible).

@@ -100,9 +100,12 @@ int dev_expand_half(hipStream_t s, int32_t n, const int32_t* nop, const int32_t*
 
 // Verlet list of `kind` from device positions: pair predicate
 // r2 = (dx*dx + dy*dy) + dz*dz < search^2, unfused, minimum image for
-// box > 0; rows ascending in i, partners ascending.
+// box > 0; rows ascending in i. With sort_rows, partners ascending (the
+// reference's order); without, each row's partners come in an order fixed
+// by the positions (stencil runs, cells, particle index within a cell), the
+// same on every build from the same positions.
 int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double search_length,
-                 int kind, ListWork& w, DevCSR& out);
+                 int kind, ListWork& w, DevCSR& out, bool sort_rows = true);
 
 // Smallest and largest of a[0..n) (synchronises); 0, 0 for n = 0.
 int dev_minmax(hipStream_t s, const int32_t* a, int32_t n, ListWork& w, int32_t* mn, int32_t* mx);

@@ -453,6 +453,42 @@ __global__ void k_cell_scatter(int32_t n, PosIn q, const int32_t* __restrict__ c
   }
 }
 
+// Each cell's particles by ascending index (insertion sort, one thread per
+// cell): the scatter's atomic slots leave them in arrival order.
+__global__ void k_cell_order(int64_t ncell, const int32_t* __restrict__ cell_start,
+                             double4* __restrict__ cpos) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < ncell;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t a = cell_start[c], b = cell_start[c + 1];
+    for (int32_t k = a + 1; k < b; k++) {
+      const double4 v = cpos[k];
+      int32_t m = k;
+      while (m > a && cpos[m - 1].w > v.w) {
+        cpos[m] = cpos[m - 1];
+        m--;
+      }
+      cpos[m] = v;
+    }
+  }
+}
+
+// Row r of a slab (src + r * stride, len[r] entries) to dst + start[r], in
+// order; one wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_copy_slab(int32_t n, const int32_t* __restrict__ len,
+                                                      const int32_t* __restrict__ start,
+                                                      const int32_t* __restrict__ src,
+                                                      int64_t stride, int32_t* __restrict__ dst) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6); r < n;
+       r += waves) {
+    const int32_t m = len[r];
+    const int32_t* in = src + r * stride;
+    int32_t* o = dst + start[r];
+    for (int32_t k = lane; k < m; k += 64) o[k] = in[k];
+  }
+}
+
 // The reference's pair predicate (cpu_ref/force_aos.cpp:131-135) with the
 // minimum image of the periodic box: every operation rounded separately, in
 // the reference's order, (dx*dx + dy*dy) + dz*dz < SL2.
@@ -537,6 +573,139 @@ __global__ __launch_bounds__(kBlock) void k_pairs(int32_t n, const double4* __re
   if (MODE != 1) nop[i] = cnt;
 }
 
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// One wavefront per cell, for cell grids with a stencil (R >= 1). The cell's
+// particles sit one per lane (in groups of 64); the candidates of the
+// (2R+1)^3 cells around it are walked as one flattened sequence, 64 per
+// round, so each candidate is loaded once for all of the cell's particles
+// instead of once per particle. A round's hits for one particle go to
+// consecutive addresses of its row (ballot, then each hit lane's rank among
+// them), so the stores coalesce. The runs of the flattened sequence are the
+// x-rows of the stencil, contiguous in cell order: (2R+1)^2 rows, each split
+// at most once where it wraps around a periodic box (the grid has at least
+// 2R+1 cells per side). Same predicate (within_search, in the same argument
+// order), exclusions and MODEs as k_pairs; a row's entries come out in
+// candidate order and are sorted afterwards.
+template <int MODE, bool HALF, int R>
+__global__ __launch_bounds__(kBlock) void k_pairs_wave(int32_t ncell,
+                                                       const double4* __restrict__ cpos,
+                                                       const int32_t* __restrict__ cell_start,
+                                                       GridArgs g, double sl2, int32_t cap,
+                                                       int32_t* __restrict__ nop,
+                                                       const int32_t* __restrict__ ptr,
+                                                       int32_t* __restrict__ out) {
+  constexpr int kSpan = 2 * R + 1;
+  constexpr int kRuns = 2 * kSpan * kSpan;
+  static_assert(kRuns <= 64, "one run per lane");
+  const int lane = threadIdx.x & 63;
+  const bool periodic = g.box > 0.0;
+  const double half_box = 0.5 * g.box;
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+  for (int64_t c64 = (blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x) >> 6; c64 < ncell;
+       c64 += waves) {
+    const int32_t c = static_cast<int32_t>(c64);
+    const int32_t p0 = cell_start[c], p1 = cell_start[c + 1];
+    if (p0 == p1) continue;  // the whole wavefront
+    const int cx = c % g.nc0, cy = (c / g.nc0) % g.nc1, cz = c / (g.nc0 * g.nc1);
+    // Lane r < kRuns: run r of the stencil as [rs, rs + rl) in cpos.
+    int32_t rs = 0, rl = 0;
+    if (lane < kRuns) {
+      const int yz = lane >> 1;
+      int z = cz + yz / kSpan - R, y = cy + yz % kSpan - R;
+      bool ok = true;
+      if (periodic) {
+        z = (z + g.nc2) % g.nc2;
+        y = (y + g.nc1) % g.nc1;
+      } else {
+        ok = z >= 0 && z < g.nc2 && y >= 0 && y < g.nc1;
+      }
+      const int x0 = cx - R, x1 = cx + R;
+      int a = 1, b = 0;  // inclusive x range of cells; empty when a > b
+      if ((lane & 1) == 0) {
+        a = max(x0, 0);
+        b = min(x1, g.nc0 - 1);
+      } else if (periodic && x0 < 0) {
+        a = x0 + g.nc0;
+        b = g.nc0 - 1;
+      } else if (periodic && x1 >= g.nc0) {
+        a = 0;
+        b = x1 - g.nc0;
+      }
+      if (ok && a <= b) {
+        const int32_t rowc = (z * g.nc1 + y) * g.nc0;
+        rs = cell_start[rowc + a];
+        rl = cell_start[rowc + b + 1] - rs;
+      }
+    }
+    int32_t pre = rl;  // inclusive prefix of the run lengths
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(pre, o, 64);
+      if (lane >= o) pre += v;
+    }
+    const int32_t excl = pre - rl;
+    const int32_t total = __shfl(pre, 63, 64);
+    for (int32_t g0 = p0; g0 < p1; g0 += 64) {
+      const int32_t mg = min(64, p1 - g0);
+      double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (lane < mg) pi = cpos[g0 + lane];
+      const int32_t iv = static_cast<int32_t>(pi.w);
+      int64_t basev = 0;
+      if (MODE == 1) basev = lane < mg ? ptr[iv] : 0;
+      if (MODE == 2) basev = static_cast<int64_t>(iv) * cap;
+      int32_t cntv = 0;
+      for (int32_t f0 = 0; f0 < total; f0 += 64) {
+        const int32_t f = f0 + lane;
+        const bool live = f < total;
+        // The run holding f: the number of runs ending at or before f.
+        int r = 0;
+#pragma unroll
+        for (int bit = 32; bit > 0; bit >>= 1) {
+          const int t = r + bit;
+          const int32_t v = __shfl(pre, min(t, 64) - 1, 64);
+          if (t <= kRuns && v <= f) r = t;
+        }
+        const int rr = min(r, kRuns - 1);
+        const int32_t u = __shfl(rs, rr, 64) + (f - __shfl(excl, rr, 64));
+        // Lanes past the end read candidate 0 (a valid address) and test nothing.
+        const double4 pj = cpos[live ? u : 0];
+        const int32_t j = static_cast<int32_t>(pj.w);
+        for (int k = 0; k < mg; k++) {
+          double4 a;
+          a.x = readlane_d(pi.x, k);
+          a.y = readlane_d(pi.y, k);
+          a.z = readlane_d(pi.z, k);
+          a.w = 0.0;
+          const int32_t ik = __builtin_amdgcn_readlane(iv, k);
+          const bool hit = live && (HALF ? j > ik : j != ik) &&
+                           within_search(a, pj, g.box, half_box, sl2);
+          const uint64_t mask = __ballot(hit);
+          if (mask == 0) continue;
+          const int32_t ck = __builtin_amdgcn_readlane(cntv, k);
+          if (MODE != 0 && hit) {
+            const int32_t rank = static_cast<int32_t>(__builtin_amdgcn_mbcnt_hi(
+                static_cast<uint32_t>(mask >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+            const int64_t bk =
+                (static_cast<int64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(
+                     static_cast<int32_t>(basev >> 32), k)))
+                 << 32) |
+                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(basev), k));
+            if (MODE == 1 || ck + rank < cap) out[bk + ck + rank] = j;
+          }
+          if (lane == k) cntv = ck + __popcll(mask);
+        }
+      }
+      if (MODE != 1 && lane < mg) nop[iv] = cntv;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // LDS tiles
 // ---------------------------------------------------------------------------
@@ -545,6 +714,7 @@ constexpr int kTileHash = 1 << kTileHashBits;          // open-addressing table 
 constexpr int kTileHashLimit = 3072;                   // distinct keys before giving up
 constexpr int kTileMaxStage = kTileLdsBudget / 24 + 1; // staged positions that can fit
 constexpr int kTileUniq = 4096;                        // sort buffer (power of two)
+constexpr int kTileBatch = 8;                          // list entries loaded ahead per thread
 static_assert(kTileMaxStage <= kTileHashLimit, "the hash limit must cover every tile that fits");
 static_assert(kTileHashLimit + kBlock < kTileHash, "probing must always find a free slot");
 static_assert(kTileHashLimit <= kTileUniq, "the sort buffer must hold every distinct key");
@@ -562,18 +732,30 @@ __device__ int32_t tile_insert(int32_t i0, int32_t ni, const int32_t* __restrict
   if (threadIdx.x == 0) *counter = 0;
   __syncthreads();
   const int64_t m = ni + static_cast<int64_t>(ne);
-  for (int64_t t = threadIdx.x; t < m; t += kBlock) {
-    const int32_t key = t < ni ? i0 + static_cast<int32_t>(t) : list[e0 + (t - ni)];
-    uint32_t h = tile_hash(key);
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <=
-           kTileHashLimit) {
-      const int32_t prev = atomicCAS(&table[h], -1, key);
-      if (prev == -1) {
-        atomicAdd(counter, 1);
-        break;
+  // Keys are loaded kTileBatch at a time before any is inserted, so each
+  // thread waits for memory once per batch rather than once per key.
+  for (int64_t t0 = 0; t0 < m; t0 += kTileBatch * kBlock) {
+    int32_t keys[kTileBatch];
+#pragma unroll
+    for (int r = 0; r < kTileBatch; r++) {
+      const int64_t t = t0 + r * kBlock + threadIdx.x;
+      keys[r] = t >= m ? -1 : t < ni ? i0 + static_cast<int32_t>(t) : list[e0 + (t - ni)];
+    }
+#pragma unroll
+    for (int r = 0; r < kTileBatch; r++) {
+      const int32_t key = keys[r];
+      if (key < 0) continue;
+      uint32_t h = tile_hash(key);
+      while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <=
+             kTileHashLimit) {
+        const int32_t prev = atomicCAS(&table[h], -1, key);
+        if (prev == -1) {
+          atomicAdd(counter, 1);
+          break;
+        }
+        if (prev == key) break;
+        h = (h + 1) & (kTileHash - 1);
       }
-      if (prev == key) break;
-      h = (h + 1) & (kTileHash - 1);
     }
   }
   __syncthreads();
@@ -587,6 +769,60 @@ __device__ __forceinline__ int32_t tile_find(const int32_t* table, int32_t key) 
   uint32_t h = tile_hash(key);
   while (table[h] != key) h = (h + 1) & (kTileHash - 1);
   return static_cast<int32_t>(h);
+}
+
+// Ascending sort of uniq[0 .. kBlock * R) by the workgroup's wavefronts:
+// element e = w*64R + r*64 + lane sits in register r of lane `lane` of
+// wavefront w. Bitonic network: exchanges closer than 64 go through lane
+// shuffles, closer than 64R between a lane's own registers, and only the
+// farther ones (log2(waves) stages per merge size beyond 64R) through LDS.
+template <int R>
+__device__ void block_bitonic(int32_t* uniq) {
+  constexpr int kSpan = 64 * R;
+  constexpr int P = kBlock * R;
+  const int lane = threadIdx.x & 63;
+  const int wbase = static_cast<int>(threadIdx.x >> 6) * kSpan;
+  int32_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = uniq[wbase + r * 64 + lane];
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= kSpan) {
+#pragma unroll
+        for (int r = 0; r < R; r++) uniq[wbase + r * 64 + lane] = v[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int e = wbase + r * 64 + lane;
+          const int32_t other = uniq[e ^ j];
+          const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
+          v[r] = keep_min ? min(v[r], other) : max(v[r], other);
+        }
+        __syncthreads();
+      } else if (j >= 64) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int pr = r ^ (j / 64);
+          if (pr > r) {
+            const bool up = ((wbase + r * 64 + lane) & k) == 0;
+            const int32_t a = v[r], c = v[pr];
+            v[r] = up ? min(a, c) : max(a, c);
+            v[pr] = up ? max(a, c) : min(a, c);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int32_t other = __shfl_xor(v[r], j, 64);
+          const bool up = ((wbase + r * 64 + lane) & k) == 0;
+          const bool keep_min = ((lane & j) == 0) == up;
+          v[r] = keep_min ? min(v[r], other) : max(v[r], other);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) uniq[wbase + r * 64 + lane] = v[r];
 }
 
 struct TileGeom {
@@ -658,23 +894,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
     int64_t off = block_exclusive(cnt, &total, sh);
     for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++)
       if (table[t] != -1) uniq[off++] = table[t];
-    int32_t P = 1;
+    int32_t P = kBlock;
     while (P < ns) P <<= 1;
     for (int32_t t = ns + threadIdx.x; t < P; t += kBlock) uniq[t] = INT32_MAX;
     __syncthreads();
-    for (int32_t k = 2; k <= P; k <<= 1)
-      for (int32_t j = k >> 1; j > 0; j >>= 1) {
-        for (int32_t t = threadIdx.x; t < P / 2; t += kBlock) {
-          const int32_t a = (t / j) * 2 * j + (t % j), c = a + j;
-          const bool up = (a & k) == 0;
-          const int32_t x = uniq[a], y = uniq[c];
-          if ((x > y) == up) {
-            uniq[a] = y;
-            uniq[c] = x;
-          }
-        }
-        __syncthreads();
-      }
+    switch (P / kBlock) {
+      case 1: block_bitonic<1>(uniq); break;
+      case 2: block_bitonic<2>(uniq); break;
+      case 4: block_bitonic<4>(uniq); break;
+      case 8: block_bitonic<8>(uniq); break;
+      default: block_bitonic<16>(uniq); break;
+    }
+    __syncthreads();
     const int64_t s0 = soff[b];
     for (int32_t t = threadIdx.x; t < ns; t += kBlock) {
       const int32_t key = uniq[t];
@@ -693,9 +924,18 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
       d[7] = 0;
     }
     __syncthreads();
-    for (int32_t e = threadIdx.x; e < g.ne; e += kBlock) {
-      const int32_t j = list[g.e0 + e];
-      local[g.e0 + e] = static_cast<uint16_t>(rank_of[tile_find(table, j)] * 24);
+    for (int32_t e0 = 0; e0 < g.ne; e0 += kTileBatch * kBlock) {
+      int32_t js[kTileBatch];
+#pragma unroll
+      for (int r = 0; r < kTileBatch; r++) {
+        const int32_t e = e0 + r * kBlock + threadIdx.x;
+        js[r] = e < g.ne ? list[g.e0 + e] : -1;
+      }
+#pragma unroll
+      for (int r = 0; r < kTileBatch; r++) {
+        const int32_t e = e0 + r * kBlock + threadIdx.x;
+        if (js[r] >= 0) local[g.e0 + e] = static_cast<uint16_t>(rank_of[tile_find(table, js[r])] * 24);
+      }
     }
     __syncthreads();
   }
@@ -891,7 +1131,7 @@ int dev_expand_half(hipStream_t s, int32_t n, const int32_t* hnop, const int32_t
 }
 
 int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double search_length,
-                 int kind, ListWork& w, DevCSR& out) {
+                 int kind, ListWork& w, DevCSR& out, bool sort_rows) {
   const double sl2 = search_length * search_length;
   out.n = n;
   LCHECK(dev_reserve(out.nop, sizeof(int32_t) * std::max(n, 1), s));
@@ -955,23 +1195,38 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   LTRY(hipGetLastError(), "cell scatter kernel");
   const bool half = kind == LJGPU_LIST_HALF;
   const int R = cg.brute ? 0 : (fine ? 2 : 1);
+  // Unsorted rows need a candidate order fixed by the positions alone: the
+  // stencil kernel's run order with each cell's particles by index. The
+  // single-cell kernel has no such order, so it always sorts.
+  if (R == 0) sort_rows = true;
+  if (!sort_rows) {
+    k_cell_order<<<grid_for(ncell, kBlock), kBlock, 0, s>>>(ncell, cstart, w.cpos.as<double4>());
+    LTRY(hipGetLastError(), "cell order kernel");
+  }
   const int grid = grid_for(n, kBlock, INT32_MAX);
+  const int wave_grid = grid_for(ncell * 64, kBlock);  // one wavefront per cell, grid-stride
   auto pairs = [&](auto mode_tag, int32_t cap, int32_t* nop, const int32_t* ptr, int32_t* outl) {
     constexpr int M = decltype(mode_tag)::value;
     const double4* cp = w.cpos.as<double4>();
     const int32_t* cs = w.cell_sorted.as<int32_t>();
-#define LJ_PAIRS(H, RR) \
-  k_pairs<M, H, RR><<<grid, kBlock, 0, s>>>(n, cp, cs, cstart, g, sl2, cap, nop, ptr, outl)
+    // A single all-particle cell (R = 0) runs one thread per particle; a
+    // stencil grid one wavefront per cell.
+#define LJ_PAIRS(H) \
+  k_pairs<M, H, 0><<<grid, kBlock, 0, s>>>(n, cp, cs, cstart, g, sl2, cap, nop, ptr, outl)
+#define LJ_PAIRS_WAVE(H, RR)                                                                     \
+  k_pairs_wave<M, H, RR><<<wave_grid, kBlock, 0, s>>>(static_cast<int32_t>(ncell), cp, cstart, g, \
+                                                      sl2, cap, nop, ptr, outl)
     if (half) {
-      if (R == 0) LJ_PAIRS(true, 0);
-      else if (R == 1) LJ_PAIRS(true, 1);
-      else LJ_PAIRS(true, 2);
+      if (R == 0) LJ_PAIRS(true);
+      else if (R == 1) LJ_PAIRS_WAVE(true, 1);
+      else LJ_PAIRS_WAVE(true, 2);
     } else {
-      if (R == 0) LJ_PAIRS(false, 0);
-      else if (R == 1) LJ_PAIRS(false, 1);
-      else LJ_PAIRS(false, 2);
+      if (R == 0) LJ_PAIRS(false);
+      else if (R == 1) LJ_PAIRS_WAVE(false, 1);
+      else LJ_PAIRS_WAVE(false, 2);
     }
 #undef LJ_PAIRS
+#undef LJ_PAIRS_WAVE
   };
   // Single pass into a slab of `cap` entries per row, sized from the last
   // build's longest row (or from the mean density), then one sort from the
@@ -1006,10 +1261,23 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   LCHECK(dev_reserve(out.list, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
   if (cap > 0 && mx <= cap) {
     w.last_max_row = mx;
+    if (!sort_rows) {
+      k_copy_slab<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(
+          n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(), cap,
+          out.list.as<int32_t>());
+      LTRY(hipGetLastError(), "slab copy kernel");
+      return LJGPU_OK;
+    }
     return row_sort(s, n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(),
                     out.list.as<int32_t>(), cap);
   }
   if (cap > 0) w.last_max_row = mx;
+  if (!sort_rows) {
+    pairs(std::integral_constant<int, 1>{}, 0, out.nop.as<int32_t>(), out.ptr.as<int32_t>(),
+          out.list.as<int32_t>());
+    LTRY(hipGetLastError(), "pair fill kernel");
+    return LJGPU_OK;
+  }
   LCHECK(dev_reserve(w.tmp, sizeof(int32_t) * std::max<int64_t>(total, 1), s));
   pairs(std::integral_constant<int, 1>{}, 0, out.nop.as<int32_t>(), out.ptr.as<int32_t>(),
         w.tmp.as<int32_t>());

@@ -255,7 +255,9 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   TRY(dev_reserve(ctx->md_qref, sizeof(double) * 4 * std::max(n, 1), ctx->stream));
   auto rebuild = [&]() -> int {
     HIP_TRY(hipEventRecord(b0, ctx->stream), "hipEventRecord");
-    TRY(build_list(ctx, in, n, box, search, LJGPU_LIST_FULL));
+    // Rows in the build's own order: the dynamics need the pair set, not
+    // the reference's row order, and that order is fixed by the positions.
+    TRY(build_list(ctx, in, n, box, search, LJGPU_LIST_FULL, false));
     if (n > 0)
       HIP_TRY(hipMemcpyAsync(ctx->md_qref.p, q, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice,
                              ctx->stream), "hipMemcpyAsync");
