@@ -454,7 +454,9 @@ __global__ void k_cell_scatter(int32_t n, PosIn q, const int32_t* __restrict__ c
 }
 
 // Each cell's particles by ascending index (insertion sort, one thread per
-// cell): the scatter's atomic slots leave them in arrival order.
+// cell): the scatter's atomic slots leave them in arrival order. Used only
+// when no cell holds more than kCellOrderMax particles.
+constexpr int32_t kCellOrderMax = 256;
 __global__ void k_cell_order(int64_t ncell, const int32_t* __restrict__ cell_start,
                              double4* __restrict__ cpos) {
   for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < ncell;
@@ -1199,6 +1201,13 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   // stencil kernel's run order with each cell's particles by index. The
   // single-cell kernel has no such order, so it always sorts.
   if (R == 0) sort_rows = true;
+  if (!sort_rows) {
+    // The per-cell insertion sort is quadratic in the cell's population:
+    // crowded cells (a collapsed or clustered system) sort rows instead.
+    int32_t cmin = 0, cmax = 0;
+    LCHECK(dev_minmax(s, count, static_cast<int32_t>(ncell), w, &cmin, &cmax));
+    if (cmax > kCellOrderMax) sort_rows = true;
+  }
   if (!sort_rows) {
     k_cell_order<<<grid_for(ncell, kBlock), kBlock, 0, s>>>(ncell, cstart, w.cpos.as<double4>());
     LTRY(hipGetLastError(), "cell order kernel");

@@ -253,3 +253,32 @@ def test_md_zero_steps_changes_nothing(ctx):
     np.testing.assert_array_equal(dq.download(), q0)
     np.testing.assert_array_equal(dp.download(), p0)
     assert st.samples == 1 and samples.shape[0] == 1
+
+
+def test_md_crowded_cell_falls_back_to_sorted_rows(ctx):
+    """A cell holding more particles than the per-cell ordering handles
+    (here 294 on a 7x7x6 lattice of spacing 0.1) makes the MD rebuild sort
+    its rows; the run is still deterministic and its list the exact full
+    list. The time step is so small that no position changes (the largest
+    force is about 1e15, so a drift moves a particle by about 1e-25), so both
+    kicks of the one step use the list of the starting positions."""
+    rng = np.random.default_rng(3)
+    box = 20.0
+    g = np.stack(np.meshgrid(np.arange(7), np.arange(7), np.arange(6), indexing="ij"), -1)
+    crowd = 5.0 + 0.1 * g.reshape(-1, 3)
+    spread = box * rng.random((706, 3))
+    q = np.zeros((1000, 4))
+    q[:, :3] = np.concatenate([crowd, spread])
+    p = np.zeros_like(q)
+    out = []
+    for _ in range(2):
+        dq, dp = ctx.array(q), ctx.array(p)
+        st, _ = ctx.md_run(dq, dp, box, 1, 1e-20, skin=0.3, rebuild_every=1)
+        out.append((dq.download(), dp.download(), st.pair_entries, st.rebuilds))
+    np.testing.assert_array_equal(out[0][0], q)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.all(np.isfinite(out[0][1])) and np.abs(out[0][1]).max() > 0
+    full = G.makepair(q, box, G.LIST_FULL, search_length=3.3)
+    assert out[0][3] == 2  # the first build and the one at step 1
+    assert out[0][2] == 2 * full.n_entries  # two kicks over the same list
