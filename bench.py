@@ -62,7 +62,7 @@ def cpu_model() -> str:
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lj_force_f64_N119164_rho1_traffic.json")
 
 
-def measured_traffic(info):
+def measured_traffic(info, tile_order):
     """HBM bytes per launch of the FP64 force kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE runs (copy-calibrated) when they were
     taken with the kernel configuration in use; None otherwise."""
@@ -72,8 +72,9 @@ def measured_traffic(info):
     except (OSError, ValueError):
         return None
     cfg = t.get("kernel_config", {})
-    if (cfg.get("lanes_per_particle"), cfg.get("unroll"), cfg.get("tile_particles")) != \
-            (info.lanes_per_particle, info.unroll, info.tile_particles):
+    if (cfg.get("lanes_per_particle"), cfg.get("unroll"), cfg.get("tile_particles"),
+            cfg.get("tile_order")) != (info.lanes_per_particle, info.unroll,
+                                       info.tile_particles, tile_order):
         return None
     return t.get("hbm_bytes_per_launch")
 
@@ -255,6 +256,8 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
     ap.add_argument("--unroll", type=int, default=0, help="entries per lane per batch (0: default)")
     ap.add_argument("--tile", type=int, default=-1, help="particles per LDS tile (-1: default)")
+    ap.add_argument("--tile-order", choices=["spatial", "index"], default="spatial",
+                    help="order of the particles in LDS tiles")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
@@ -303,8 +306,8 @@ def main() -> int:
     ctx.set_kernel_config(args.lanes, args.block, args.unroll)
     if args.tile >= 0:
         ctx.set_tiling(args.tile)
+    ctx.set_tile_order(G.TILE_ORDER_INDEX if args.tile_order == "index" else G.TILE_ORDER_SPATIAL)
     ctx.set_list(full)
-    info = ctx.list_info()
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)
 
@@ -330,6 +333,9 @@ def main() -> int:
         mixed(args.warmup)
         ms_mixed = timed(mixed, args.steps)
         log(f"mixed: {args.steps} steps in {ms_mixed:.3f} ms")
+    # The tiling in use (spatially ordered at the first evaluation).
+    info = ctx.list_info()
+    tile_order = ("spatial" if ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL else "index")
 
     # Force error of the mixed kernel against the FP64 kernel (one step, dt = 1).
     dF = ctx.zeros(q.shape)
@@ -370,6 +376,7 @@ def main() -> int:
             "unroll": info.unroll,
             "tile_particles": info.tile_particles,
             "tile_max_stage": info.tile_max_stage,
+            "tile_order": tile_order,
         },
         "roofline": {
             "bound": "hbm",
@@ -377,7 +384,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(info),
+            "traffic": measured_traffic(info, tile_order),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": round(per_launch_s * 1e6, 3),
             "flop_rate_tflops": round(FLOP_PER_ENTRY * e / per_launch_s / 1e12, 3),

@@ -72,6 +72,10 @@ extern "C" {
 #define LJGPU_HALF_EXPAND 2     /* expand to the full list once at
                                    set-list time; deterministic             */
 
+/* Order of the particles in LDS tiles (ljgpu_set_tile_order). */
+#define LJGPU_TILE_ORDER_INDEX 0    /* blocks of consecutive indices        */
+#define LJGPU_TILE_ORDER_SPATIAL 1  /* blocks of nearby particles (default) */
+
 typedef struct ljgpu_ctx ljgpu_ctx;
 
 typedef struct ljgpu_list_info {
@@ -186,9 +190,10 @@ int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode);
 int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes_per_particle, int block_size, int unroll);
 
 /* LDS tiling of full lists (on by default, 64 particles per tile): the list
- * is regrouped once per ljgpu_set_list_* into blocks of consecutive
- * particles whose partners' positions are staged in LDS, and each entry is
- * re-expressed as a 16-bit LDS offset. 0 turns tiling off (global-gather
+ * is regrouped once per list into blocks of particles whose partners'
+ * positions are staged in LDS, and each entry is re-expressed as a 16-bit
+ * LDS offset. The particles of a block are consecutive in the order chosen
+ * by ljgpu_set_tile_order. 0 turns tiling off (global-gather
  * kernels). The tile actually used is capped at 1024 / lanes_per_particle
  * particles (one workgroup per tile); ELL lists and half lists evaluated
  * with atomics are never tiled. stage_offsets = 1 also stages each tile's
@@ -210,6 +215,26 @@ int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
 int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_t* stage,
                     int64_t stage_capacity, uint16_t* local, int64_t local_capacity,
                     int32_t* n_blocks, int64_t* n_stage);
+
+/* Order of the particles in tiles. LJGPU_TILE_ORDER_SPATIAL (the default)
+ * sorts each periodic system into columns along x, about 0.75 *
+ * cbrt(tile_particles / density) wide, and each column into x bins of about
+ * one particle, ordered by (column, bin, index). A block of consecutive
+ * particles in that order is a compact group that stages fewer partner
+ * positions than a block of consecutive indices. The order needs positions:
+ * ljgpu_build_list takes it from its own, a list installed with
+ * ljgpu_set_list_* from the first force evaluation (index order until
+ * then). Open boxes, and bins of more than 256 particles, keep index order.
+ * Sums run over each row in its own order either way, so the order changes
+ * results only by the rounding of the periodic unwrap. No reference
+ * counterpart (a device layout choice). */
+int ljgpu_set_tile_order(ljgpu_ctx* ctx, int order);
+
+/* The order of the tiling in use (*order = LJGPU_TILE_ORDER_INDEX or
+ * _SPATIAL) and, when member is not null, the particle in each slot
+ * (particle_number values). Derives the tiling first, like ljgpu_get_tiles;
+ * *order = -1 when the kernels run untiled. Inspection only. */
+int ljgpu_get_tile_order(ljgpu_ctx* ctx, int* order, int32_t* member, int64_t capacity);
 
 /* Autotuner. Times `reps` force evaluations (after as many warm-up ones)
  * over the installed list on device AoS positions q for each candidate
@@ -389,16 +414,30 @@ int ljgpu_make_transposed(int32_t particle_number, const int32_t* number_of_part
  * number_of_partners[i], as sortpair produces): the tiled counterpart of
  * make_transposed_pairlist, cuda/force_cuda.cu:229-240. desc receives 8 ints
  * per block (first particle, particle count, staging offset, staging count,
- * first list entry, entry count, replica (0 here), 0), stage the staged
- * particle indices,
+ * first list entry, entry count, replica (0 here), middle particle = first
+ * particle + count / 2), stage the staged particle indices,
  * local (n_entries values, parallel to sorted_list) the LDS byte offset
- * 24*slot of each entry's j within its block's staging area; max_stage the
+ * 24*rank of each entry's j within its block's staging area; max_stage the
  * largest staging count. Pass null outputs to query the sizes first. */
 int ljgpu_make_tiles(int32_t particle_number, const int32_t* number_of_partners,
                      const int32_t* pointer, const int32_t* sorted_list, int64_t n_entries,
                      int32_t tile_particles, int32_t* desc, int64_t desc_capacity,
                      int32_t* stage, int64_t stage_capacity, uint16_t* local,
                      int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage);
+
+/* As ljgpu_make_tiles, with the particles taken in the order `member` (a
+ * permutation of 0..particle_number-1, one particle per slot; blocks cover
+ * consecutive slots): desc's first particle and first entry become the
+ * block's first slot and its first entry in the rows stored in slot order,
+ * each block's staged particles are sorted by slot instead of index, and
+ * local follows the rows in slot order. member = null is ljgpu_make_tiles. No
+ * reference counterpart: the host model of ljgpu_get_tile_order's tiling. */
+int ljgpu_make_tiles_ordered(int32_t particle_number, const int32_t* number_of_partners,
+                             const int32_t* pointer, const int32_t* sorted_list,
+                             int64_t n_entries, int32_t tile_particles, const int32_t* member,
+                             int32_t* desc, int64_t desc_capacity, int32_t* stage,
+                             int64_t stage_capacity, uint16_t* local, int32_t* n_blocks,
+                             int64_t* n_stage, int32_t* max_stage);
 
 /* ---- pair-list caches (host files, no device needed) ---------------------- */
 /* pair.dat, the binary cache of the CPU programs (savepair / loadpair,

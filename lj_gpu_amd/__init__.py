@@ -53,6 +53,8 @@ ACCUM_KAHAN_F32 = 1
 HALF_AUTO = 0
 HALF_ATOMIC = 1
 HALF_EXPAND = 2
+TILE_ORDER_INDEX = 0
+TILE_ORDER_SPATIAL = 1
 
 # The reference problem's constants (cpu_ref/force_aos.cpp:10-14,31-33).
 CUTOFF_LENGTH = 3.0
@@ -159,6 +161,10 @@ def lib() -> C.CDLL:
         "ljgpu_make_transposed": [_i32, _vp, _vp, _vp, _i32, _vp],
         "ljgpu_make_tiles": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _i64, _vp, _i64, _vp,
                              C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_make_tiles_ordered": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64,
+                                     _vp, C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_set_tile_order": [_vp, C.c_int],
+        "ljgpu_get_tile_order": [_vp, C.POINTER(C.c_int), _vp, _i64],
         "ljgpu_build_list": [_vp, _vp, _i32, _d, _d, C.c_int],
         "ljgpu_build_list_soa": [_vp, _vp, _vp, _vp, _i32, _d, _d, C.c_int],
         "ljgpu_get_tiles": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, C.POINTER(_i32),
@@ -370,24 +376,28 @@ def read_pair_cache(path: str, particle_number: int, kind: int) -> CSR:
 @dataclass
 class Tiles:
     """LDS tiling of a full list (what ljgpu_set_tiling installs)."""
-    desc: np.ndarray      # (n_blocks, 8): first particle, count, staging offset,
-                          # staging count, first entry, entry count, 0, 0
+    desc: np.ndarray      # (n_blocks, 8): first slot, count, staging offset,
+                          # staging count, first entry, entry count, replica,
+                          # middle particle (of slot first + count // 2)
     stage: np.ndarray     # staged particle indices
-    local: np.ndarray     # uint16 LDS byte offsets, parallel to sorted_list
+    local: np.ndarray     # uint16 LDS byte offsets of the rows in slot order
     max_stage: int
 
 
-def make_tiles(csr: CSR, tile_particles: int = 64) -> Tiles:
+def make_tiles(csr: CSR, tile_particles: int = 64, member=None) -> Tiles:
+    """Host model of the LDS tiling; member (one particle per slot) orders
+    the particles the blocks cover, None for index order."""
     nb, ns, ms = _i32(0), _i64(0), _i32(0)
+    mem = None if member is None else np.ascontiguousarray(member, dtype=np.int32)
     args = (csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer), _ptr(csr.sorted_list),
-            csr.n_entries, tile_particles)
-    _check(lib().ljgpu_make_tiles(*args, None, 0, None, 0, None, C.byref(nb), C.byref(ns),
-                                  C.byref(ms)))
+            csr.n_entries, tile_particles, None if mem is None else _ptr(mem))
+    _check(lib().ljgpu_make_tiles_ordered(*args, None, 0, None, 0, None, C.byref(nb),
+                                          C.byref(ns), C.byref(ms)))
     desc = np.zeros(max(8 * nb.value, 1), np.int32)
     stage = np.zeros(max(ns.value, 1), np.int32)
     local = np.zeros(max(csr.n_entries, 1), np.uint16)
-    _check(lib().ljgpu_make_tiles(*args, _ptr(desc), desc.size, _ptr(stage), stage.size,
-                                  _ptr(local), C.byref(nb), C.byref(ns), C.byref(ms)))
+    _check(lib().ljgpu_make_tiles_ordered(*args, _ptr(desc), desc.size, _ptr(stage), stage.size,
+                                          _ptr(local), C.byref(nb), C.byref(ns), C.byref(ms)))
     return Tiles(desc[: 8 * nb.value].reshape(-1, 8), stage[: ns.value],
                  local[: csr.n_entries], ms.value)
 
@@ -533,6 +543,22 @@ class Context:
                                      _ptr(local), local.size, C.byref(nb), C.byref(ns)))
         d = desc.reshape(-1, 8)
         return Tiles(d, stage[: ns.value], local[:e], int(d[:, 3].max()))
+
+    def set_tile_order(self, order: int) -> None:
+        """TILE_ORDER_SPATIAL (default): tiles of nearby particles;
+        TILE_ORDER_INDEX: tiles of consecutive indices."""
+        _check(lib().ljgpu_set_tile_order(self.handle, order))
+
+    def get_tile_order(self):
+        """(order in use, particle of each slot), or (-1, None) when untiled."""
+        order = C.c_int(0)
+        _check(lib().ljgpu_get_tile_order(self.handle, C.byref(order), None, 0))
+        if order.value < 0:
+            return -1, None
+        n = self.list_info().particle_number
+        member = np.zeros(max(n, 1), np.int32)
+        _check(lib().ljgpu_get_tile_order(self.handle, C.byref(order), _ptr(member), member.size))
+        return order.value, member[:n]
 
     def set_replicas(self, offsets, boxes) -> None:
         """Batched replicas (offsets: n_replicas + 1 particle offsets; boxes:

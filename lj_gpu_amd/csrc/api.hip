@@ -33,6 +33,7 @@ void reset_list(ljgpu_ctx* ctx) {
   ctx->ell_ready = false;
   ctx->ell_width = 0;
   ctx->tiles_for = 0;
+  ctx->tiles_ordered = false;
   ctx->tiles.tile_particles = 0;
   ctx->tiles.blocks = 0;
 }
@@ -44,6 +45,8 @@ void release_all(ljgpu_ctx* ctx) {
   dev_release(ctx->ell);
   dev_release(ctx->tiles);
   dev_release(ctx->work);
+  dev_release(ctx->slot_q);
+  dev_release(ctx->slot_p);
 }
 
 // Particles per tile for the current kernel configuration: the requested
@@ -68,22 +71,50 @@ int ensure_ell(ljgpu_ctx* ctx) {
 // Derive the LDS tiling of the evaluated full list for the current
 // configuration. Where a block does not fit the LDS budget the tile size is
 // halved; a list whose single rows do not fit keeps the untiled kernels.
-int ensure_tiles(ljgpu_ctx* ctx) {
+// With the spatial tile order the blocks take the particles in the order of
+// dev_tile_order, which needs positions: a list built on the device gets it
+// at once, a list installed from the caller at its first force evaluation
+// (index order until then). Systems without a periodic box stay in index
+// order.
+int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
   const int32_t want = ctx->eval_half ? 0 : effective_tile(ctx);
-  if (want == ctx->tiles_for) return LJGPU_OK;
+  const bool spatial = ctx->tile_order == LJGPU_TILE_ORDER_SPATIAL && want > 0;
+  if (want == ctx->tiles_for && (!spatial || ctx->tiles_ordered || !q)) return LJGPU_OK;
   ctx->tiles.tile_particles = 0;
   ctx->tiles.blocks = 0;
+  bool ordered = false;
+  if (spatial && q && ctx->eval.n > 0)
+    TRY(dev_tile_order(ctx->stream, *q, ctx->eval.n, box, ctx->rep_off, ctx->rep_boxes, want,
+                       ctx->work, &ordered));
+  const int32_t* member = ordered ? ctx->work.order.as<int32_t>() : nullptr;
   for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
-    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, ctx->work, ctx->tiles));
+    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, member, ctx->work, ctx->tiles));
     if (ctx->tiles.tile_particles > 0) break;
   }
   ctx->tiles_for = want;
+  ctx->tiles_ordered = spatial && q;
   return LJGPU_OK;
 }
 
-int prepare_layouts(ljgpu_ctx* ctx) {
+int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q, double box) {
   if (ctx->lanes == 1) return ensure_ell(ctx);
-  return ensure_tiles(ctx);
+  return ensure_tiles(ctx, q, box);
+}
+
+PosIn positions_of(const ForceArgs& fa) {
+  PosIn in;
+  if (fa.soa) {
+    in.x = fa.q[0];
+    in.y = fa.q[1];
+    in.z = fa.q[2];
+    in.stride = 1;
+  } else {
+    in.x = fa.q[0];
+    in.y = fa.q[0] + 1;
+    in.z = fa.q[0] + 2;
+    in.stride = 4;
+  }
+  return in;
 }
 
 // Turn a validated device CSR (the caller's, or staged from the host) into
@@ -117,7 +148,8 @@ int derive_list(ljgpu_ctx* ctx, int32_t n, const int32_t* nop, const int32_t* pt
   }
   ctx->ell_ready = false;
   ctx->tiles_for = -1;
-  TRY(prepare_layouts(ctx));
+  ctx->tiles_ordered = false;
+  TRY(prepare_layouts(ctx, nullptr, 0.0));
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }
@@ -201,7 +233,8 @@ int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double sea
                    &ctx->max_np));
   ctx->ell_ready = false;
   ctx->tiles_for = -1;
-  TRY(prepare_layouts(ctx));
+  ctx->tiles_ordered = false;
+  TRY(prepare_layouts(ctx, &q, box));
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }
@@ -221,8 +254,9 @@ int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps, doubl
 }
 
 // Kernel arguments of one force evaluation over the installed list.
-int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc) {
-  TRY(prepare_layouts(ctx));
+int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* lc) {
+  const PosIn in = positions_of(fa);
+  TRY(prepare_layouts(ctx, &in, fa.box));
   *lv = ListView();
   lv->n = ctx->n;
   lv->nop = ctx->eval.nop.as<int32_t>();
@@ -251,6 +285,10 @@ int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc) {
       lv->tile_desc = ctx->tiles.desc.as<int32_t>();
       lv->tile_stage = ctx->tiles.stage.as<int32_t>();
       lv->tile_local = ctx->tiles.local.as<uint16_t>();
+      lv->tile_member = ctx->tiles.member.as<int32_t>();
+      lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
+      if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL && ctx->tiles.staged > 0)
+        lv->tile_stage_slot = ctx->tiles.stage_slot.as<int32_t>();
     }
   }
   return LJGPU_OK;
@@ -267,7 +305,7 @@ int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum) {
   if (ctx->n == 0) return LJGPU_OK;
   ListView lv;
   LaunchConfig lc;
-  TRY(force_view(ctx, &lv, &lc));
+  TRY(force_view(ctx, fa, &lv, &lc));
   return launch_one(ctx, lv, lc, fa, mixed, accum);
 }
 
@@ -295,12 +333,103 @@ int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* ou
   return LJGPU_OK;
 }
 
-int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, int accum) {
+namespace {
+
+// Copies between the caller's arrays and slot order: out[s] = in[member[s]]
+// (gather) or out[member[s]] = in[s] (scatter), 4-double AoS rows or three
+// SoA arrays.
+template <bool GATHER>
+__global__ void k_slot_copy_aos(int32_t n, const int32_t* __restrict__ member,
+                                const double4* __restrict__ in, double4* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = member[t];
+    if (GATHER) out[t] = in[i];
+    else out[i] = in[t];
+  }
+}
+
+template <bool GATHER>
+__global__ void k_slot_copy_soa(int32_t n, const int32_t* __restrict__ member, const double* x,
+                                const double* y, const double* z, double* ox, double* oy,
+                                double* oz) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = member[t];
+    const int64_t a = GATHER ? i : t, b = GATHER ? t : i;
+    ox[b] = x[a];
+    oy[b] = y[a];
+    oz[b] = z[a];
+  }
+}
+
+int slot_grid(int32_t n) { return static_cast<int>(std::min<int64_t>((n + 255) / 256, 4096)); }
+
+// Positions and momenta of `fa` copied into the context's slot-order
+// buffers; *out addresses the copies.
+int slot_gather(ljgpu_ctx* ctx, const ForceArgs& fa, ForceArgs* out) {
+  const int32_t n = ctx->n;
+  const int32_t* mem = ctx->tiles.member.as<int32_t>();
+  const size_t bytes = sizeof(double) * 4 * static_cast<size_t>(n);
+  TRY(dev_reserve(ctx->slot_q, bytes, ctx->stream));
+  TRY(dev_reserve(ctx->slot_p, bytes, ctx->stream));
+  *out = fa;
+  double* q = ctx->slot_q.as<double>();
+  double* p = ctx->slot_p.as<double>();
+  if (!fa.soa) {
+    k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
+        n, mem, reinterpret_cast<const double4*>(fa.q[0]), reinterpret_cast<double4*>(q));
+    k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
+        n, mem, reinterpret_cast<const double4*>(fa.p[0]), reinterpret_cast<double4*>(p));
+    out->q[0] = q;
+    out->p[0] = p;
+  } else {
+    k_slot_copy_soa<true><<<slot_grid(n), 256, 0, ctx->stream>>>(n, mem, fa.q[0], fa.q[1], fa.q[2],
+                                                                 q, q + n, q + 2 * n);
+    k_slot_copy_soa<true><<<slot_grid(n), 256, 0, ctx->stream>>>(n, mem, fa.p[0], fa.p[1], fa.p[2],
+                                                                 p, p + n, p + 2 * n);
+    for (int c = 0; c < 3; c++) {
+      out->q[c] = q + static_cast<int64_t>(c) * n;
+      out->p[c] = p + static_cast<int64_t>(c) * n;
+    }
+  }
+  HIP_TRY(hipGetLastError(), "slot gather kernels");
+  return LJGPU_OK;
+}
+
+// Momenta in slot order (slot) back to the caller's arrays (fa).
+int slot_scatter(ljgpu_ctx* ctx, const ForceArgs& slot, const ForceArgs& fa) {
+  const int32_t n = ctx->n;
+  const int32_t* mem = ctx->tiles.member.as<int32_t>();
+  if (!fa.soa)
+    k_slot_copy_aos<false><<<slot_grid(n), 256, 0, ctx->stream>>>(
+        n, mem, reinterpret_cast<const double4*>(slot.p[0]), reinterpret_cast<double4*>(fa.p[0]));
+  else
+    k_slot_copy_soa<false><<<slot_grid(n), 256, 0, ctx->stream>>>(
+        n, mem, slot.p[0], slot.p[1], slot.p[2], fa.p[0], fa.p[1], fa.p[2]);
+  HIP_TRY(hipGetLastError(), "slot scatter kernel");
+  return LJGPU_OK;
+}
+
+}  // namespace
+
+int run_steps(ljgpu_ctx* ctx, const ForceArgs& caller, int32_t steps, bool mixed, int accum) {
   TRY(use_device(ctx));
   if (ctx->n == 0 || steps == 0) return LJGPU_OK;
   ListView lv;
   LaunchConfig lc;
-  TRY(force_view(ctx, &lv, &lc));
+  TRY(force_view(ctx, caller, &lv, &lc));
+  // With spatially ordered tiles, a call of many steps runs over copies of
+  // q and p in slot order (q does not change between its steps): the rows'
+  // and the staging's accesses are then contiguous runs. The arithmetic is
+  // the same, so the momenta equal those of per-step evaluations bit for bit.
+  ForceArgs fa = caller;
+  const bool slots = steps >= kSlotMinSteps && lv.tile_blocks > 0 && lv.tile_stage_slot;
+  if (slots) {
+    TRY(slot_gather(ctx, caller, &fa));
+    lv.tile_member = nullptr;
+    lv.tile_stage = lv.tile_stage_slot;
+  }
   int32_t done = 0;
   if (ctx->use_graphs && steps >= kGraphMinSteps) {
     // Repeated evaluations run as replays of one captured graph of up to
@@ -329,6 +458,7 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, in
       HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream), "hipGraphLaunch");
   }
   for (; done < steps; done++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
+  if (slots) TRY(slot_scatter(ctx, fa, caller));
   return LJGPU_OK;
 }
 
@@ -480,6 +610,18 @@ int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
   return LJGPU_OK;
 }
 
+int ljgpu_set_tile_order(ljgpu_ctx* ctx, int order) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (order != LJGPU_TILE_ORDER_INDEX && order != LJGPU_TILE_ORDER_SPATIAL)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unknown tile order");
+  if (order != ctx->tile_order) {
+    ctx->tile_order = order;
+    ctx->tiles_for = -1;  // re-derived at the next use
+    ctx->tiles_ordered = false;
+  }
+  return LJGPU_OK;
+}
+
 int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
   if (half_mode != LJGPU_HALF_AUTO && half_mode != LJGPU_HALF_ATOMIC && half_mode != LJGPU_HALF_EXPAND)
@@ -608,6 +750,22 @@ int ljgpu_get_list_csr(ljgpu_ctx* ctx, int32_t* nop, int32_t* ptr, int32_t* list
   if (c.e > 0)
     HIP_TRY(hipMemcpyAsync(list, c.list.p, sizeof(int32_t) * c.e, hipMemcpyDeviceToHost, ctx->stream),
             "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return LJGPU_OK;
+}
+
+int ljgpu_get_tile_order(ljgpu_ctx* ctx, int* order, int32_t* member, int64_t capacity) {
+  if (!ctx || !order) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  TRY(use_device(ctx));
+  TRY(prepare_layouts(ctx));
+  const DevTiles& t = ctx->tiles;
+  const bool tiled = ctx->lanes > 1 && t.blocks > 0 && !ctx->eval_half;
+  *order = tiled ? t.order : -1;
+  if (!tiled || !member) return LJGPU_OK;
+  if (capacity < ctx->eval.n) return set_error(LJGPU_ERR_CAPACITY, "get_tile_order: output too small");
+  HIP_TRY(hipMemcpyAsync(member, t.member.p, sizeof(int32_t) * ctx->eval.n, hipMemcpyDeviceToHost,
+                         ctx->stream), "hipMemcpyAsync");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }

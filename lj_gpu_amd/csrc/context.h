@@ -19,6 +19,9 @@ namespace ljgpu {
 // kGraphChunk launches.
 constexpr int32_t kGraphMinSteps = 4;
 constexpr int32_t kGraphChunk = 100;
+// Calls with at least kSlotMinSteps steps over spatially ordered tiles run
+// over copies of q and p in slot order (two gathers and a scatter per call).
+constexpr int32_t kSlotMinSteps = 16;
 
 // Every argument of a captured force loop; a graph is replayed only for an
 // identical key (compared bytewise, so it is zero-filled before use).
@@ -62,7 +65,10 @@ struct ljgpu_ctx {
   int32_t tile_resident = 0;      // persistent tile grid: workgroups per CU (0: one per tile)
   int cus = 0;                    // compute units of the device
   int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
+  int tile_order = LJGPU_TILE_ORDER_SPATIAL;  // requested order of the particles in tiles
+  bool tiles_ordered = false;     // the tiling had positions to take that order from
   ljgpu::DevTiles tiles;
+  ljgpu::DevMem slot_q, slot_p;   // q and p in slot order during a many-step call
 
   ljgpu::ListWork work;
 
@@ -111,7 +117,11 @@ inline int hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 int use_device(ljgpu_ctx* ctx);
-int prepare_layouts(ljgpu_ctx* ctx);
+// Derive the layouts the configured kernel needs. Positions (q, with the
+// periodic box, 0 for none) let the tiles take the spatial order.
+int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q = nullptr, double box = 0.0);
+// Device positions of a force evaluation's arguments.
+PosIn positions_of(const ForceArgs& fa);
 // Build the list from device positions and install it (ljgpu_build_list).
 int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double search, int kind,
                bool sort_rows = true);
@@ -121,8 +131,8 @@ int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum);
 int run_steps(ljgpu_ctx* ctx, const ForceArgs& fa, int32_t steps, bool mixed, int accum);
 void drop_graph(ljgpu_ctx* ctx);
 // Kernel arguments of one force evaluation over the installed list (derives
-// missing layouts first).
-int force_view(ljgpu_ctx* ctx, ListView* lv, LaunchConfig* lc);
+// missing layouts first, from the positions of `fa`).
+int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* lc);
 // Record the launches `body` enqueues on the context stream into an
 // executable graph.
 int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out);

@@ -329,12 +329,16 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __
                                               acc);
 }
 
-// LDS-tiled full-list kernel: one workgroup per block of consecutive
-// particles. The workgroup first stages into LDS (a) the positions of every
+// LDS-tiled full-list kernel: one workgroup per block of consecutive slots
+// of a particle order (slot s holds particle member[s]; the order groups
+// nearby particles, or is the identity). The workgroup first stages into
+// LDS (a) the positions of every
 // particle the block's rows reference (stage[soff .. soff+ns)), as x, y, z
 // triples, in a periodic box shifted to the image nearest the block's middle
 // particle, and (b) the block's ne per-entry LDS offsets (one contiguous,
-// coalesced range of `local`). Rows then run entirely out of LDS: the random
+// coalesced range of `local`, which holds the rows in slot order: slot s's
+// row starts at slot_ptr[s]). A null member means that positions, momenta
+// and `stage` are given in slot order (particle index = slot). Rows then run entirely out of LDS: the random
 // gathers of the CSR kernel become LDS reads and the list costs 2 bytes per
 // entry. In the unwrapped frame a pair difference equals its minimum image
 // whenever every particle of the block lies within box/2 - rc of the
@@ -351,7 +355,8 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __
 // arithmetic.
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
 __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restrict__ nop,
-                                          const int32_t* __restrict__ ptr,
+                                          const int32_t* __restrict__ member,
+                                          const int32_t* __restrict__ slot_ptr,
                                           const int4* __restrict__ desc,
                                           const int32_t* __restrict__ stage,
                                           const uint16_t* __restrict__ local, int b,
@@ -376,7 +381,8 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   double* lds = reinterpret_cast<double*>(lb);
   const int off_area = (ns * 24 + 15) & ~15;
   double cx = 0.0, cy = 0.0, cz = 0.0;
-  if constexpr (PBC) pos.ld(i0 + ni / 2, cx, cy, cz);
+  // The block's middle particle (its slot when the arrays are in slot order).
+  if constexpr (PBC) pos.ld(member ? d1.w : i0 + ni / 2, cx, cy, cz);
   // Staging issues all of a thread's loads of a pass before its LDS writes,
   // so each thread has kStageR independent gathers in flight.
   constexpr int kStageR = 4;
@@ -427,7 +433,8 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   const int seg = static_cast<int>(threadIdx.x) / TPI;
   const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
   const bool active = seg < ni;
-  const int i = i0 + (active ? seg : 0);
+  const int slot = i0 + (active ? seg : 0);
+  const int i = member ? member[slot] : slot;
   double xi, yi, zi;
   pos.ld(i, xi, yi, zi);
   int far = 0;
@@ -435,8 +442,9 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
     unwrap_near<PBC>(xi, yi, zi, cx, cy, cz, pp);
     far = active && (fabs(xi - cx) >= reach || fabs(yi - cy) >= reach || fabs(zi - cz) >= reach);
   }
-  const int np = active ? nop[i] : 0;
-  const int rs = active ? ptr[i] - eoff : 0;
+  const int sp = slot_ptr[slot];
+  const int np = active ? slot_ptr[slot + 1] - sp : 0;
+  const int rs = active ? sp - eoff : 0;
   // The frame check ORed over the workgroup through one LDS word: threads
   // that fail it write this tile's index, which no earlier tile of this
   // workgroup used, so the word needs no reset.
@@ -468,8 +476,9 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
 // gives five, measured 2-3 % slower.
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
-k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ ptr,
-       const int4* __restrict__ desc, const int32_t* __restrict__ stage,
+k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ member,
+       const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
+       const int32_t* __restrict__ stage,
        const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach,
        const double2* __restrict__ rep_box) {
   // Dynamic LDS only (no static LDS, so its base is address 0 and constant
@@ -485,8 +494,8 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   const int stride = static_cast<int>(gridDim.x) / kXcds;
   const int t_end = min(nblocks, (xcd + 1) * per);
   for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
-    tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, ptr, desc, stage, local, b, pp,
-                                                 reach, rep_box, lb, flag);
+    tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, member, slot_ptr, desc, stage,
+                                                 local, b, pp, reach, rep_box, lb, flag);
     __syncthreads();  // the next tile overwrites this tile's LDS
   }
 }
@@ -539,7 +548,7 @@ int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& 
                                  : static_cast<size_t>(lv.tile_max_stage) * 24);
   const double reach = 0.5 * pp.box - sqrt(pp.cl2);
   hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(grid), dim3(threads),
-                     lds_bytes, s, pos, mom, lv.nop, lv.ptr,
+                     lds_bytes, s, pos, mom, lv.nop, lv.tile_member, lv.tile_slot_ptr,
                      reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
                      lv.tile_blocks, pp, reach, reinterpret_cast<const double2*>(lv.replica_box));
   return launch_status("tiled force kernel launch");

@@ -116,7 +116,8 @@ void csr_make_row_ordered(HostCSR* c) {
   c->list.swap(list);
 }
 
-bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out) {
+bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* member,
+                 HostTiles* out) {
   const int32_t n = full.n;
   out->tile_particles = tile_particles;
   out->max_stage = 0;
@@ -125,16 +126,22 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out) {
   out->stage.clear();
   out->local.assign(full.list.size(), 0);
   if (tile_particles <= 0 || !csr_is_row_ordered(full)) return false;
-  auto entries = [&](int32_t i0, int32_t ni) -> int64_t {
-    return static_cast<int64_t>(full.ptr[i0 + ni - 1]) + full.nop[i0 + ni - 1] - full.ptr[i0];
-  };
+  auto mem = [&](int32_t s) { return member ? member[s] : s; };
+  // Blocks key their particles by slot.
+  std::vector<int32_t> slot_of(static_cast<size_t>(n));
+  for (int32_t s = 0; s < n; s++) slot_of[mem(s)] = s;
+  // Row start of each slot in the rows stored in slot order.
+  std::vector<int64_t> sptr(static_cast<size_t>(n) + 1, 0);
+  for (int32_t s = 0; s < n; s++) sptr[s + 1] = sptr[s] + full.nop[mem(s)];
+  auto entries = [&](int32_t i0, int32_t ni) -> int64_t { return sptr[i0 + ni] - sptr[i0]; };
   std::vector<int32_t> set;
-  auto collect = [&](int32_t i0, int32_t ni) {
+  auto collect = [&](int32_t i0, int32_t ni) {  // the block's slots, ascending
     set.clear();
-    for (int32_t i = i0; i < i0 + ni; i++) {
-      set.push_back(i);
+    for (int32_t s = i0; s < i0 + ni; s++) {
+      const int32_t i = mem(s);
+      set.push_back(s);
       const int32_t* r = full.list.data() + full.ptr[i];
-      set.insert(set.end(), r, r + full.nop[i]);
+      for (int32_t k = 0; k < full.nop[i]; k++) set.push_back(slot_of[r[k]]);
     }
     std::sort(set.begin(), set.end());
     set.erase(std::unique(set.begin(), set.end()), set.end());
@@ -152,18 +159,19 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out) {
     if (tile_lds_bytes(static_cast<int64_t>(set.size()), ne) > kTileLdsBudget) return false;
     const int32_t soff = static_cast<int32_t>(out->stage.size());
     const int32_t ns = static_cast<int32_t>(set.size());
-    out->desc.insert(out->desc.end(),
-                     {i0, ni, soff, ns, full.ptr[i0], static_cast<int32_t>(ne), 0, 0});
-    out->stage.insert(out->stage.end(), set.begin(), set.end());
+    out->desc.insert(out->desc.end(), {i0, ni, soff, ns, static_cast<int32_t>(sptr[i0]),
+                                       static_cast<int32_t>(ne), 0, mem(i0 + ni / 2)});
+    for (int32_t v : set) out->stage.push_back(mem(v));
     out->max_stage = std::max(out->max_stage, ns);
     out->lds_bytes = std::max(out->lds_bytes, static_cast<int32_t>(tile_lds_bytes(ns, ne)));
-    for (int32_t i = i0; i < i0 + ni; i++) {
+    for (int32_t s = i0; s < i0 + ni; s++) {
+      const int32_t i = mem(s);
       const int32_t kp = full.ptr[i];
       for (int32_t k = 0; k < full.nop[i]; k++) {
-        const int32_t j = full.list[kp + k];
-        const int32_t slot = static_cast<int32_t>(
+        const int32_t j = slot_of[full.list[kp + k]];
+        const int32_t rank = static_cast<int32_t>(
             std::lower_bound(set.begin(), set.end(), j) - set.begin());
-        out->local[kp + k] = static_cast<uint16_t>(slot * 24);
+        out->local[sptr[s] + k] = static_cast<uint16_t>(rank * 24);
       }
     }
     i0 += ni;
@@ -446,14 +454,23 @@ int ljgpu_random_shfl(int32_t n, const int32_t* nop, const int32_t* ptr, int32_t
   return LJGPU_OK;
 }
 
-int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,
-                     int64_t n_entries, int32_t tile_particles, int32_t* desc,
-                     int64_t desc_capacity, int32_t* stage, int64_t stage_capacity,
-                     uint16_t* local, int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
+int ljgpu_make_tiles_ordered(int32_t n, const int32_t* nop, const int32_t* ptr,
+                             const int32_t* list, int64_t n_entries, int32_t tile_particles,
+                             const int32_t* member, int32_t* desc, int64_t desc_capacity,
+                             int32_t* stage, int64_t stage_capacity, uint16_t* local,
+                             int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
   if (!n_blocks || !n_stage || !max_stage || tile_particles <= 0 || tile_particles > 1024)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_tiles: bad argument");
   int rc = validate_csr(n, nop, ptr, list, n_entries, LJGPU_LIST_FULL);
   if (rc) return rc;
+  if (member) {
+    std::vector<char> seen(static_cast<size_t>(n), 0);
+    for (int32_t s = 0; s < n; s++) {
+      if (member[s] < 0 || member[s] >= n || seen[member[s]])
+        return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_tiles: member is not a permutation");
+      seen[member[s]] = 1;
+    }
+  }
   HostCSR c;
   c.n = n;
   c.nop.assign(nop, nop + n);
@@ -462,7 +479,7 @@ int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const in
   if (!csr_is_row_ordered(c))
     return set_error(LJGPU_ERR_BAD_LIST, "make_tiles: rows must be stored in order");
   HostTiles t;
-  if (!build_tiles(c, tile_particles, &t))
+  if (!build_tiles(c, tile_particles, member, &t))
     return set_error(LJGPU_ERR_CAPACITY, "make_tiles: a row does not fit one tile");
   *n_blocks = static_cast<int32_t>(t.desc.size() / 8);
   *n_stage = static_cast<int64_t>(t.stage.size());
@@ -475,6 +492,15 @@ int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const in
   std::memcpy(stage, t.stage.data(), t.stage.size() * sizeof(int32_t));
   std::memcpy(local, t.local.data(), t.local.size() * sizeof(uint16_t));
   return LJGPU_OK;
+}
+
+int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,
+                     int64_t n_entries, int32_t tile_particles, int32_t* desc,
+                     int64_t desc_capacity, int32_t* stage, int64_t stage_capacity,
+                     uint16_t* local, int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
+  return ljgpu_make_tiles_ordered(n, nop, ptr, list, n_entries, tile_particles, nullptr, desc,
+                                  desc_capacity, stage, stage_capacity, local, n_blocks, n_stage,
+                                  max_stage);
 }
 
 // cuda/force_cuda.cu:229-240.

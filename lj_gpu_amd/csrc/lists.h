@@ -39,13 +39,21 @@ struct DevCSR {
 void dev_release(DevCSR& c);
 
 // LDS tiling of a row-ordered full list (the layout ListView describes).
+// Blocks cover consecutive slots of a particle order: slot s holds particle
+// member[s], whose row occupies local[slot_ptr[s] .. slot_ptr[s+1]).
 struct DevTiles {
   int32_t tile_particles = 0;  // particles per block (0: no tiling)
   int32_t blocks = 0;
   int64_t staged = 0;
   int32_t max_stage = 0;
   int32_t lds_bytes = 0;
+  int order = 0;               // LJGPU_TILE_ORDER_INDEX (member[s] = s) or _SPATIAL
   DevMem desc, stage, local;
+  DevMem member;      // int32 per slot
+  DevMem slot_ptr;    // int32 per slot (+1)
+  DevMem stage_slot;  // stage as slots (spatial order only)
+  // Each block's staged particles are sorted by slot (by index in index
+  // order), and local holds 24 * the rank of each entry's partner among them.
 };
 void dev_release(DevTiles& t);
 
@@ -65,6 +73,9 @@ struct ListWork {
   DevMem tile_soff;   // int32 per tile (+1)
   DevMem tile_geo;    // int32 x 4 per tile: first particle, count, replica
   DevMem rep_of;      // int32 per particle: replica index
+  DevMem order;       // int32 per particle: spatial tile order
+  DevMem order_grid;  // OrderGrid per system
+  DevMem slot_of;     // int32 per particle: its slot in the tile order
   DevCSR part;        // one replica's list during a batched build
   int32_t last_max_row = 0;  // longest row of the last device build (slab size hint)
 };
@@ -124,8 +135,24 @@ int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell);
 // block's replica. Returns LJGPU_OK with out.tile_particles = 0 when a block
 // does not fit the LDS budget (the caller then halves the tile size or
 // evaluates untiled).
+// `member` (device, one particle per slot, a permutation that keeps every
+// replica's particles within its own slot range) orders the particles the
+// blocks cover; null takes them in index order.
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles,
-                   const std::vector<int32_t>& replicas, ListWork& w, DevTiles& out);
+                   const std::vector<int32_t>& replicas, const int32_t* member, ListWork& w,
+                   DevTiles& out);
+
+// Spatial particle order for LDS tiles, into w.order: each periodic system
+// (the whole set, or every replica of `replicas` with its box) is cut into
+// columns along x about 0.75 * cbrt(tile_particles / density) wide in y and
+// z, each column into x bins of about one particle; particles are ordered by
+// (system, z column, y column, x bin, index). Consecutive slots then hold
+// compact groups of particles, so a block of them stages fewer partners than
+// a block of consecutive indices. *ok = false (and no order) for an open box,
+// non-finite positions or a bin of more than 256 particles.
+int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
+                   const std::vector<int32_t>& replicas, const std::vector<double>& boxes,
+                   int32_t tile_particles, ListWork& w, bool* ok);
 
 // Batched replicas: lists of every replica (particles [off[r], off[r+1]),
 // box boxes[r]) built on the device into one CSR with global indices.

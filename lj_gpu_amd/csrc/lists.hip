@@ -725,15 +725,80 @@ __device__ __forceinline__ uint32_t tile_hash(int32_t key) {
   return (static_cast<uint32_t>(key) * 2654435761u) >> (32 - kTileHashBits);
 }
 
-// Insert the block's keys (its own particles and every entry's partner) into
-// an LDS hash set. Returns the number of distinct keys, or -1 when there are
-// more than kTileHashLimit (such a block cannot fit the LDS budget).
-__device__ int32_t tile_insert(int32_t i0, int32_t ni, const int32_t* __restrict__ list, int64_t e0,
-                               int32_t ne, int32_t* table, int32_t* counter) {
+// A block's rows, staged in LDS: slot i0 + s holds particle mem[s], whose
+// row is list[src[s] ..) and covers entries [beg[s], beg[s + 1]) of the
+// block (beg[ni] = ne).
+constexpr int kTileRowsMax = 1024;
+struct TileRows {
+  int32_t mem[kTileRowsMax];
+  int32_t src[kTileRowsMax];
+  int32_t beg[kTileRowsMax + 1];
+};
+
+struct TileGeom {
+  int32_t i0, ni, ne, rid;
+  int64_t e0;
+};
+
+// Block b covers slots [geo[4b], geo[4b] + geo[4b+1]) of replica geo[4b+2];
+// its entries are [slot_ptr[i0], slot_ptr[i0 + ni]) of the slot-ordered rows.
+__device__ __forceinline__ TileGeom tile_geom(int32_t b, const int32_t* __restrict__ geo,
+                                              const int32_t* __restrict__ slot_ptr) {
+  TileGeom g;
+  g.i0 = geo[4 * b];
+  g.ni = geo[4 * b + 1];
+  g.rid = geo[4 * b + 2];
+  g.e0 = slot_ptr[g.i0];
+  g.ne = slot_ptr[g.i0 + g.ni] - static_cast<int32_t>(g.e0);
+  return g;
+}
+
+// Fill the row table of block g. Returns the list position of the block's
+// first entry when its rows lie back to back in the list (index order), so
+// entry u is list[base + u]; -1 when they are scattered (entry u is then
+// found by a binary search over beg).
+__device__ int32_t tile_rows(const TileGeom& g, const int32_t* __restrict__ member,
+                             const int32_t* __restrict__ ptr, const int32_t* __restrict__ slot_ptr,
+                             TileRows& r) {
+  for (int32_t t = threadIdx.x; t < g.ni; t += kBlock) {
+    const int32_t i = member[g.i0 + t];
+    r.mem[t] = i;
+    r.src[t] = ptr[i];
+    r.beg[t] = slot_ptr[g.i0 + t] - static_cast<int32_t>(g.e0);
+  }
+  if (threadIdx.x == 0) r.beg[g.ni] = g.ne;
+  __syncthreads();
+  const int32_t base = r.src[0];
+  int ok = 1;
+  for (int32_t t = threadIdx.x; t < g.ni; t += kBlock) ok &= (r.src[t] - r.beg[t] == base);
+  return __syncthreads_and(ok) ? base : -1;
+}
+
+// Partner of the block's entry u (0 <= u < ne).
+__device__ __forceinline__ int32_t tile_entry(const TileRows& r, int32_t ni, int32_t base,
+                                              const int32_t* __restrict__ list, int32_t u) {
+  if (base >= 0) return list[base + u];
+  int32_t lo = 0, hi = ni;  // beg[lo] <= u < beg[hi]
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (r.beg[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return list[r.src[lo] + (u - r.beg[lo])];
+}
+
+// Insert the block's keys (the slots of its own particles and of every
+// entry's partner; slot_of null: index order, slot = particle) into an LDS
+// hash set. Returns the number of distinct keys, or -1 when there are more
+// than kTileHashLimit (such a block cannot fit the LDS budget).
+__device__ int32_t tile_insert(const TileGeom& g, const TileRows& rows, int32_t base,
+                               const int32_t* __restrict__ list,
+                               const int32_t* __restrict__ slot_of, int32_t* table,
+                               int32_t* counter) {
   for (int32_t t = threadIdx.x; t < kTileHash; t += kBlock) table[t] = -1;
   if (threadIdx.x == 0) *counter = 0;
   __syncthreads();
-  const int64_t m = ni + static_cast<int64_t>(ne);
+  const int64_t m = g.ni + static_cast<int64_t>(g.ne);
   // Keys are loaded kTileBatch at a time before any is inserted, so each
   // thread waits for memory once per batch rather than once per key.
   for (int64_t t0 = 0; t0 < m; t0 += kTileBatch * kBlock) {
@@ -741,7 +806,14 @@ __device__ int32_t tile_insert(int32_t i0, int32_t ni, const int32_t* __restrict
 #pragma unroll
     for (int r = 0; r < kTileBatch; r++) {
       const int64_t t = t0 + r * kBlock + threadIdx.x;
-      keys[r] = t >= m ? -1 : t < ni ? i0 + static_cast<int32_t>(t) : list[e0 + (t - ni)];
+      keys[r] = t >= m     ? -1
+                : t < g.ni ? -2 - static_cast<int32_t>(t)  // own slot g.i0 + t, below
+                           : tile_entry(rows, g.ni, base, list, static_cast<int32_t>(t - g.ni));
+    }
+#pragma unroll
+    for (int r = 0; r < kTileBatch; r++) {
+      if (keys[r] <= -2) keys[r] = g.i0 - 2 - keys[r];
+      else if (keys[r] >= 0 && slot_of) keys[r] = slot_of[keys[r]];
     }
 #pragma unroll
     for (int r = 0; r < kTileBatch; r++) {
@@ -827,38 +899,23 @@ __device__ void block_bitonic(int32_t* uniq) {
   for (int r = 0; r < R; r++) uniq[wbase + r * 64 + lane] = v[r];
 }
 
-struct TileGeom {
-  int32_t i0, ni, ne, rid;
-  int64_t e0;
-};
-
-// Block b covers particles [geo[4b], geo[4b] + geo[4b+1]) of replica
-// geo[4b+2]; its entries are one contiguous range of the row-ordered list.
-__device__ __forceinline__ TileGeom tile_geom(int32_t b, const int32_t* __restrict__ geo,
-                                              const int32_t* __restrict__ nop,
-                                              const int32_t* __restrict__ ptr) {
-  TileGeom g;
-  g.i0 = geo[4 * b];
-  g.ni = geo[4 * b + 1];
-  g.rid = geo[4 * b + 2];
-  g.e0 = ptr[g.i0];
-  g.ne = static_cast<int32_t>(static_cast<int64_t>(ptr[g.i0 + g.ni - 1]) + nop[g.i0 + g.ni - 1] - g.e0);
-  return g;
-}
-
 // Pass 1: distinct staged particles per block; flags blocks that exceed the
 // LDS budget. stats: [0] overflow, [1] max staged, [2] max LDS bytes.
 __global__ __launch_bounds__(kBlock) void k_tile_count(const int32_t* __restrict__ geo, int32_t nblocks,
-                                                       const int32_t* __restrict__ nop,
+                                                       const int32_t* __restrict__ member,
+                                                       const int32_t* __restrict__ slot_of,
+                                                       const int32_t* __restrict__ slot_ptr,
                                                        const int32_t* __restrict__ ptr,
                                                        const int32_t* __restrict__ list,
                                                        int32_t* __restrict__ ns_out,
                                                        int32_t* __restrict__ stats) {
   __shared__ int32_t table[kTileHash];
+  __shared__ TileRows rows;
   __shared__ int32_t counter;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const TileGeom g = tile_geom(b, geo, nop, ptr);
-    const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
+    const TileGeom g = tile_geom(b, geo, slot_ptr);
+    const int32_t base = tile_rows(g, member, ptr, slot_ptr, rows);
+    const int32_t ns = tile_insert(g, rows, base, list, slot_of, table, &counter);
     if (threadIdx.x == 0) {
       ns_out[b] = ns < 0 ? 0 : ns;
       const int64_t bytes = ns < 0 ? INT32_MAX : tile_lds_bytes(ns, g.ne);
@@ -873,22 +930,27 @@ __global__ __launch_bounds__(kBlock) void k_tile_count(const int32_t* __restrict
 // block's staged particles; each entry's LDS offset is 24 * the rank of its
 // partner, looked up through the hash table.
 __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict__ geo, int32_t nblocks,
-                                                      const int32_t* __restrict__ nop,
+                                                      const int32_t* __restrict__ member,
+                                                      const int32_t* __restrict__ slot_of,
+                                                      const int32_t* __restrict__ slot_ptr,
                                                       const int32_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ list,
                                                       const int32_t* __restrict__ soff,
                                                       int32_t* __restrict__ desc,
                                                       int32_t* __restrict__ stage,
+                                                      int32_t* __restrict__ stage_slot,
                                                       uint16_t* __restrict__ local) {
   __shared__ int32_t table[kTileHash];
   __shared__ uint16_t rank_of[kTileHash];
   __shared__ int32_t uniq[kTileUniq];
+  __shared__ TileRows rows;
   __shared__ int32_t counter;
   __shared__ int64_t sh[kBlock / 64];
   constexpr int32_t C = kTileHash / kBlock;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const TileGeom g = tile_geom(b, geo, nop, ptr);
-    const int32_t ns = tile_insert(g.i0, g.ni, list, g.e0, g.ne, table, &counter);
+    const TileGeom g = tile_geom(b, geo, slot_ptr);
+    const int32_t base = tile_rows(g, member, ptr, slot_ptr, rows);
+    const int32_t ns = tile_insert(g, rows, base, list, slot_of, table, &counter);
     // Compact the occupied slots (C per thread) into uniq[0..ns), pad, sort.
     int32_t cnt = 0;
     for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++) cnt += table[t] != -1;
@@ -909,9 +971,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
     }
     __syncthreads();
     const int64_t s0 = soff[b];
+    // Staged particles in slot order: stage holds the particles, stage_slot
+    // (spatial order only) their slots.
     for (int32_t t = threadIdx.x; t < ns; t += kBlock) {
       const int32_t key = uniq[t];
-      stage[s0 + t] = key;
+      stage[s0 + t] = slot_of ? member[key] : key;
+      if (stage_slot) stage_slot[s0 + t] = key;
       rank_of[tile_find(table, key)] = static_cast<uint16_t>(t);
     }
     if (threadIdx.x == 0) {
@@ -923,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
       d[4] = static_cast<int32_t>(g.e0);
       d[5] = g.ne;
       d[6] = g.rid;
-      d[7] = 0;
+      d[7] = rows.mem[g.ni / 2];  // middle particle: the periodic unwrap's reference
     }
     __syncthreads();
     for (int32_t e0 = 0; e0 < g.ne; e0 += kTileBatch * kBlock) {
@@ -931,7 +996,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
 #pragma unroll
       for (int r = 0; r < kTileBatch; r++) {
         const int32_t e = e0 + r * kBlock + threadIdx.x;
-        js[r] = e < g.ne ? list[g.e0 + e] : -1;
+        js[r] = e < g.ne ? tile_entry(rows, g.ni, base, list, e) : -1;
+      }
+      if (slot_of) {
+#pragma unroll
+        for (int r = 0; r < kTileBatch; r++)
+          if (js[r] >= 0) js[r] = slot_of[js[r]];
       }
 #pragma unroll
       for (int r = 0; r < kTileBatch; r++) {
@@ -980,6 +1050,79 @@ __global__ void k_check_replicas(int32_t n, const int32_t* __restrict__ nop,
   }
 }
 
+// Spatial tile order: bins of one system (dev_tile_order).
+struct OrderGrid {
+  double w[3];     // bin widths: x bins, y and z columns
+  double box;      // period
+  int32_t nc[3];   // x bins, y columns, z columns
+  int32_t key0;    // first key of the system
+};
+
+__global__ void k_order_key(int32_t n, PosIn q, const OrderGrid* __restrict__ grids,
+                            const int32_t* __restrict__ rep_of, int32_t* __restrict__ count,
+                            int32_t* __restrict__ slot, int32_t* __restrict__ key_of) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const OrderGrid g = grids[rep_of ? rep_of[t] : 0];
+    const int64_t o = t * q.stride;
+    const int cx = cell_coord(q.x[o], 0.0, g.w[0], g.nc[0], g.box);
+    const int cy = cell_coord(q.y[o], 0.0, g.w[1], g.nc[1], g.box);
+    const int cz = cell_coord(q.z[o], 0.0, g.w[2], g.nc[2], g.box);
+    const int32_t k = g.key0 + (cz * g.nc[1] + cy) * g.nc[0] + cx;
+    key_of[t] = k;
+    slot[t] = atomicAdd(&count[k], 1);
+  }
+}
+
+__global__ void k_order_scatter(int32_t n, const int32_t* __restrict__ key_of,
+                                const int32_t* __restrict__ slot,
+                                const int32_t* __restrict__ start, int32_t* __restrict__ order) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    order[start[key_of[t]] + slot[t]] = static_cast<int32_t>(t);
+}
+
+// Each bin's particles by ascending index (insertion sort, one thread per
+// bin; bins hold at most kCellOrderMax particles).
+__global__ void k_order_sort(int64_t nkey, const int32_t* __restrict__ start,
+                             int32_t* __restrict__ order) {
+  for (int64_t c = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; c < nkey;
+       c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t a = start[c], b = start[c + 1];
+    for (int32_t k = a + 1; k < b; k++) {
+      const int32_t v = order[k];
+      int32_t m = k;
+      while (m > a && order[m - 1] > v) {
+        order[m] = order[m - 1];
+        m--;
+      }
+      order[m] = v;
+    }
+  }
+}
+
+__global__ void k_iota(int32_t n, int32_t* __restrict__ a) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    a[t] = static_cast<int32_t>(t);
+}
+
+// slot_of[member[s]] = s.
+__global__ void k_inverse(int32_t n, const int32_t* __restrict__ member,
+                          int32_t* __restrict__ slot_of) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    slot_of[member[t]] = static_cast<int32_t>(t);
+}
+
+// Row length of the particle in each slot.
+__global__ void k_slot_nop(int32_t n, const int32_t* __restrict__ member,
+                           const int32_t* __restrict__ nop, int32_t* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[t] = nop[member[t]];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1018,6 +1161,9 @@ void dev_release(DevTiles& t) {
   dev_release(t.desc);
   dev_release(t.stage);
   dev_release(t.local);
+  dev_release(t.member);
+  dev_release(t.slot_ptr);
+  dev_release(t.stage_slot);
   t.tile_particles = 0;
   t.blocks = 0;
   t.staged = 0;
@@ -1028,7 +1174,7 @@ void dev_release(DevTiles& t) {
 void dev_release(ListWork& w) {
   for (DevMem* m : {&w.scan_sums, &w.small, &w.cell_count, &w.cell_start, &w.cell_slot, &w.cell_of,
                     &w.cell_sorted, &w.cpos, &w.counts, &w.tmp, &w.tile_ns, &w.tile_soff,
-                    &w.tile_geo, &w.rep_of})
+                    &w.tile_geo, &w.rep_of, &w.order, &w.order_grid, &w.slot_of})
     dev_release(*m);
   dev_release(w.part);
 }
@@ -1338,12 +1484,112 @@ int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell) {
   return LJGPU_OK;
 }
 
+int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
+                   const std::vector<int32_t>& replicas, const std::vector<double>& boxes,
+                   int32_t T, ListWork& w, bool* ok) {
+  *ok = false;
+  if (n <= 0 || T <= 0) return LJGPU_OK;
+  const std::vector<int32_t> single = {0, n};
+  const std::vector<int32_t>& off = replicas.empty() ? single : replicas;
+  const int32_t nsys = static_cast<int32_t>(off.size()) - 1;
+  std::vector<OrderGrid> grids(nsys);
+  int64_t keys = 0;
+  for (int32_t r = 0; r < nsys; r++) {
+    const double L = replicas.empty() ? box : boxes[r];
+    if (!(L > 0.0) || !std::isfinite(L)) return LJGPU_OK;  // open systems keep index order
+    const int32_t m = std::max(off[r + 1] - off[r], 1);
+    OrderGrid& g = grids[r];
+    g.box = L;
+    const double rho = m / (L * L * L);
+    const double side = 0.75 * std::cbrt(T / rho);
+    // Rounded to the nearest count: for lattices of 4k^3 particles L / side
+    // often lands exactly on an integer, where truncation would follow
+    // last-bit rounding.
+    for (int c = 1; c < 3; c++) {
+      const double v = std::floor(L / side + 0.5);
+      g.nc[c] = v >= 1024.0 ? 1024 : std::max(1, static_cast<int>(v));
+    }
+    g.nc[0] = std::max<int32_t>(1, m / (g.nc[1] * g.nc[2]));
+    for (int c = 0; c < 3; c++) g.w[c] = L / g.nc[c];
+    g.key0 = static_cast<int32_t>(keys);
+    keys += static_cast<int64_t>(g.nc[0]) * g.nc[1] * g.nc[2];
+    if (keys > INT32_MAX - 1) return LJGPU_OK;
+  }
+  const size_t gbytes = (sizeof(OrderGrid) * grids.size() + 255) / 256 * 256;
+  LCHECK(dev_reserve(w.order_grid, gbytes + sizeof(int32_t) * off.size(), s));
+  LTRY(hipMemcpyAsync(w.order_grid.p, grids.data(), sizeof(OrderGrid) * grids.size(),
+                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  const int32_t* rep_of = nullptr;
+  if (!replicas.empty()) {
+    int32_t* doff = reinterpret_cast<int32_t*>(static_cast<char*>(w.order_grid.p) + gbytes);
+    LTRY(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, s),
+         "hipMemcpyAsync");
+    LCHECK(dev_reserve(w.rep_of, sizeof(int32_t) * n, s));
+    k_replica_of<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, doff, nsys, w.rep_of.as<int32_t>());
+    LTRY(hipGetLastError(), "replica index kernel");
+    rep_of = w.rep_of.as<int32_t>();
+  }
+  LCHECK(dev_reserve(w.cell_count, sizeof(int32_t) * (keys + 1), s));
+  LCHECK(dev_reserve(w.cell_start, sizeof(int32_t) * (keys + 1), s));
+  LCHECK(dev_reserve(w.cell_slot, sizeof(int32_t) * n, s));
+  LCHECK(dev_reserve(w.cell_of, sizeof(int32_t) * n, s));
+  LCHECK(dev_reserve(w.order, sizeof(int32_t) * n, s));
+  int32_t* count = w.cell_count.as<int32_t>();
+  LCHECK(fill<int32_t>(s, count, keys, 0));
+  k_order_key<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+      n, q, static_cast<const OrderGrid*>(w.order_grid.p), rep_of, count,
+      w.cell_slot.as<int32_t>(), w.cell_of.as<int32_t>());
+  LTRY(hipGetLastError(), "tile order key kernel");
+  int32_t kmin = 0, kmax = 0;
+  LCHECK(dev_minmax(s, count, static_cast<int32_t>(keys), w, &kmin, &kmax));
+  if (kmax > kCellOrderMax) return LJGPU_OK;  // a crowded bin: keep index order
+  int64_t total = 0;
+  LCHECK(dev_scan(s, count, w.cell_start.as<int32_t>(), keys, w, &total));
+  k_order_scatter<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, w.cell_of.as<int32_t>(),
+                                                         w.cell_slot.as<int32_t>(),
+                                                         w.cell_start.as<int32_t>(),
+                                                         w.order.as<int32_t>());
+  k_order_sort<<<grid_for(keys, kBlock), kBlock, 0, s>>>(keys, w.cell_start.as<int32_t>(),
+                                                         w.order.as<int32_t>());
+  LTRY(hipGetLastError(), "tile order kernels");
+  *ok = true;
+  return LJGPU_OK;
+}
+
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<int32_t>& reps,
-                   ListWork& w, DevTiles& out) {
+                   const int32_t* member, ListWork& w, DevTiles& out) {
   out.tile_particles = 0;
   out.blocks = 0;
-  if (T <= 0 || c.n <= 0) return LJGPU_OK;
-  // Blocks of T consecutive particles, restarted at every replica boundary.
+  if (T <= 0 || T > kTileRowsMax || c.n <= 0) return LJGPU_OK;
+  // The slot order and each slot's row start in the slot-ordered rows.
+  LCHECK(dev_reserve(out.member, sizeof(int32_t) * c.n, s));
+  LCHECK(dev_reserve(out.slot_ptr, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
+  int32_t* mem = out.member.as<int32_t>();
+  if (member) {
+    LTRY(hipMemcpyAsync(mem, member, sizeof(int32_t) * c.n, hipMemcpyDeviceToDevice, s),
+         "hipMemcpyAsync");
+  } else {
+    k_iota<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem);
+    LTRY(hipGetLastError(), "iota kernel");
+  }
+  out.order = member ? LJGPU_TILE_ORDER_SPATIAL : LJGPU_TILE_ORDER_INDEX;
+  // Blocks key their particles by slot: slot_of[member[s]] = s.
+  const int32_t* slot_of = nullptr;
+  if (member) {
+    LCHECK(dev_reserve(w.slot_of, sizeof(int32_t) * c.n, s));
+    k_inverse<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem, w.slot_of.as<int32_t>());
+    LTRY(hipGetLastError(), "inverse order kernel");
+    slot_of = w.slot_of.as<int32_t>();
+  }
+  LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
+  k_slot_nop<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem, c.nop.as<int32_t>(),
+                                                       w.counts.as<int32_t>());
+  LTRY(hipGetLastError(), "slot row length kernel");
+  int64_t e = 0;
+  LCHECK(dev_scan(s, w.counts.as<int32_t>(), out.slot_ptr.as<int32_t>(), c.n, w, &e));
+  if (e != c.e) return set_error(LJGPU_ERR_HIP, "tile order is not a permutation of the rows");
+  const int32_t* sptr = out.slot_ptr.as<int32_t>();
+  // Blocks of T consecutive slots, restarted at every replica boundary.
   std::vector<int32_t> geo;
   const std::vector<int32_t> single = {0, c.n};
   const std::vector<int32_t>& off = reps.empty() ? single : reps;
@@ -1362,7 +1608,7 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
   int32_t* stats = w.small.as<int32_t>() + 4;
   LCHECK(fill<int32_t>(s, stats, 3, 0));
   const int grid = std::min(nb, 65535);
-  k_tile_count<<<grid, kBlock, 0, s>>>(dgeo, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+  k_tile_count<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
                                        c.list.as<int32_t>(), w.tile_ns.as<int32_t>(), stats);
   LTRY(hipGetLastError(), "tile count kernel");
   int32_t h[3];
@@ -1378,9 +1624,13 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
   const int64_t nlocal = c.e + kTileLocalPad;
   LCHECK(dev_reserve(out.local, sizeof(uint16_t) * nlocal, s));
   LCHECK(fill<uint16_t>(s, out.local.as<uint16_t>() + c.e, kTileLocalPad, 0));
-  k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, c.nop.as<int32_t>(), c.ptr.as<int32_t>(),
+  // The staged particles as slots too, for evaluations over copies of the
+  // positions and momenta in slot order.
+  if (member) LCHECK(dev_reserve(out.stage_slot, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
+  k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
                                       c.list.as<int32_t>(), w.tile_soff.as<int32_t>(),
                                       out.desc.as<int32_t>(), out.stage.as<int32_t>(),
+                                      member ? out.stage_slot.as<int32_t>() : nullptr,
                                       out.local.as<uint16_t>());
   LTRY(hipGetLastError(), "tile fill kernel");
   out.tile_particles = T;

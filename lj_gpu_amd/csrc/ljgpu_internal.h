@@ -46,19 +46,25 @@ struct ListView {
   int64_t n_entries = 0;          // CSR length (for clamping)
   int32_t ell_width = 0;          // >0: `list` is ELL
 
-  // LDS-tiled form of a row-ordered CSR list (tile_blocks > 0 selects the
-  // tile kernel): block b covers particles [i0, i0+ni), whose entries are the
-  // contiguous list range [eoff, eoff+ne), and stages in LDS the positions
-  // of the ns particles stage[soff .. soff+ns) and the ne entry offsets.
-  // `local` is parallel to the CSR list and holds, per entry, the LDS byte
-  // offset (slot * 24) of its j in the block's position area.
+  // LDS-tiled form of a CSR list (tile_blocks > 0 selects the tile kernel):
+  // block b covers slots [i0, i0+ni) of a particle order (slot s holds
+  // particle tile_member[s]), whose entries are [eoff, eoff+ne) of the rows
+  // stored in slot order (slot s's row at tile_slot_ptr[s]), and stages in
+  // LDS the positions of the ns particles stage[soff .. soff+ns) and the ne
+  // entry offsets. `local` holds, per entry of the slot-ordered rows, the LDS
+  // byte offset (24 * rank) of its j in the block's position area.
   int32_t tile_blocks = 0;
   int32_t tile_particles = 0;       // max particles per block
   int32_t tile_lds_bytes = 0;       // max LDS bytes per block (positions + offsets)
   int32_t tile_max_stage = 0;       // max staged positions per block
-  const int32_t* tile_desc = nullptr;   // 8 ints per block: i0, ni, soff, ns, eoff, ne, 0, 0
+  const int32_t* tile_desc = nullptr;   // 8 ints per block: i0, ni, soff, ns, eoff, ne, replica,
+                                        // middle particle (member[i0 + ni/2])
   const int32_t* tile_stage = nullptr;  // staged particle indices
   const uint16_t* tile_local = nullptr; // per-entry LDS byte offsets
+  const int32_t* tile_member = nullptr;   // particle of each slot; null: positions and
+                                          // momenta are given in slot order
+  const int32_t* tile_slot_ptr = nullptr; // row start of each slot (+1)
+  const int32_t* tile_stage_slot = nullptr;  // stage as slots (spatial order; else null)
 
   // Batched replicas (tiled kernel only): per replica (box, 1/box), indexed
   // by descriptor slot 6; null for a single system.
@@ -109,10 +115,13 @@ bool csr_is_row_ordered(const HostCSR& c);
 // Rewrite `c` so that its rows are stored in order (pointer = exclusive scan).
 void csr_make_row_ordered(HostCSR* c);
 
-// Split the particles of a row-ordered list into blocks of at most
-// `tile_particles` consecutive rows that fit the LDS budget (blocks shrink
-// where needed). Returns false if a single row does not fit.
-bool build_tiles(const HostCSR& full, int32_t tile_particles, HostTiles* out);
+// Split the particles of a row-ordered list, taken in the order `member`
+// (one particle per slot; null: index order), into blocks of at most
+// `tile_particles` consecutive slots that fit the LDS budget (blocks shrink
+// where needed). local is indexed by the rows stored in slot order. Returns
+// false if a single row does not fit.
+bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* member,
+                 HostTiles* out);
 
 struct ForceArgs {
   bool soa = false;

@@ -406,7 +406,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     fa.dt = dt;
     StepKey key;
     std::memset(&key, 0, sizeof(key));
-    TRY(force_view(ctx, &key.lv, &key.lc));
+    TRY(force_view(ctx, fa, &key.lv, &key.lc));
     key.fa = fa;
     key.mixed = mixed;
     key.accum = accum;

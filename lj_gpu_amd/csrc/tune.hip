@@ -123,7 +123,8 @@ int ljgpu_autotune(ljgpu_ctx* ctx, const double* q, double box, double cl2, int 
   ctx->unroll = best->unroll;
   ctx->tile_particles = best->tile_particles;
   ctx->tile_resident = best->resident_per_cu;
-  TRY(prepare_layouts(ctx));
+  const PosIn in = positions_of(fa);
+  TRY(prepare_layouts(ctx, &in, box));
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }

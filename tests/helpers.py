@@ -77,3 +77,35 @@ def reference_mode(density: float = 0.5):
 
 def to_soa(q4: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(q4[:, :3].T)
+
+
+def column_order(q: np.ndarray, box: float, tile_particles: int = 64, offsets=None,
+                 boxes=None) -> np.ndarray:
+    """Numpy restatement of the library's spatial tile order: each periodic
+    system cut into columns along x about 0.75 * cbrt(T / density) wide in y
+    and z (the count per side rounded to the nearest integer), each column into x bins of about one particle (the bin of a
+    coordinate is floor(wrapped / width), clamped), particles ordered by
+    (system, z column, y column, x bin, index)."""
+    n = q.shape[0]
+    off = [0, n] if offsets is None else [int(v) for v in offsets]
+    bx = [box] if offsets is None else [float(v) for v in boxes]
+    keys = np.zeros(n, np.int64)
+    key0 = 0
+    for r in range(len(off) - 1):
+        a, b, L = off[r], off[r + 1], bx[r]
+        m = max(b - a, 1)
+        rho = m / (L * L * L)
+        side = 0.75 * np.cbrt(tile_particles / rho)
+        nc = [1, 1, 1]
+        for c in (1, 2):
+            v = np.floor(L / side + 0.5)
+            nc[c] = 1024 if v >= 1024.0 else max(1, int(v))
+        nc[0] = max(1, m // (nc[1] * nc[2]))
+        cell = []
+        for c in range(3):
+            x = q[a:b, c] - L * np.floor(q[a:b, c] / L)
+            v = np.floor(x / (L / nc[c])).astype(np.int64)
+            cell.append(np.clip(v, 0, nc[c] - 1))
+        keys[a:b] = key0 + (cell[2] * nc[1] + cell[1]) * nc[0] + cell[0]
+        key0 += nc[0] * nc[1] * nc[2]
+    return np.lexsort((np.arange(n), keys)).astype(np.int32)

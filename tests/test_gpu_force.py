@@ -140,14 +140,23 @@ def test_f64_tiles_in_small_periodic_box(ctx):
         assert normwise(p, p_ref) < FP64_TOL
 
 
-def test_f64_tiles_with_permuted_particle_order(ctx):
-    """Random particle order: tiles shrink to fit LDS; results unchanged."""
+@pytest.mark.parametrize("order", [G.TILE_ORDER_INDEX, G.TILE_ORDER_SPATIAL])
+def test_f64_tiles_with_permuted_particle_order(ctx, order):
+    """Random particle order: index-order tiles shrink to fit LDS, the
+    spatial order regroups the particles into full compact tiles; results
+    unchanged either way."""
     q, box, _, _ = periodic(12, 1.0)
     perm = np.random.default_rng(5).permutation(q.shape[0])
     qp = np.ascontiguousarray(q[perm])
     full = G.makepair(qp, box, G.LIST_FULL)
+    ctx.set_tile_order(order)
     p = run_aos(ctx, qp, full, box, lanes=8, unroll=2, tile=64)
-    assert ctx.list_info().tile_blocks > (full.n + 63) // 64
+    blocks = ctx.list_info().tile_blocks
+    ctx.set_tile_order(G.TILE_ORDER_SPATIAL)
+    if order == G.TILE_ORDER_INDEX:
+        assert blocks > (full.n + 63) // 64
+    else:
+        assert blocks == (full.n + 63) // 64
     assert normwise(p, oracle_momenta(12, 1.0)[perm]) < FP64_TOL
 
 
@@ -355,3 +364,47 @@ def test_bad_lists_are_rejected_before_any_launch(ctx):
     with pytest.raises(G.LJGPUError) as e:
         ctx.set_kernel_config(16, 100)
     assert e.value.code == G.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("layout", ["aos", "soa"])
+@pytest.mark.parametrize("prec", ["f64", G.ACCUM_F64, G.ACCUM_KAHAN_F32])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_slot_order_copies_are_bitwise_per_step(ctx, layout, prec, graphs):
+    """A call of 20 steps over spatially ordered tiles runs over copies of q
+    and p in slot order; 20 calls of one step run over the caller's arrays
+    through the slot order. Same arithmetic: the momenta agree bit for bit
+    (and the caller's positions are untouched)."""
+    q, box, _, full = periodic(12, 1.0)
+    ctx.set_graphs(graphs)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    res = []
+    for calls, steps in ((1, 20), (20, 1)):
+        if layout == "aos":
+            dq, dp = ctx.array(q), ctx.zeros(q.shape)
+        else:
+            q3 = to_soa(q)
+            dq = [ctx.array(np.ascontiguousarray(q3[c])) for c in range(3)]
+            dp = [ctx.zeros(q.shape[0]) for _ in range(3)]
+        for _ in range(calls):
+            if layout == "aos" and prec == "f64":
+                ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, steps)
+            elif layout == "aos":
+                ctx.force_aos_mixed(dq, dp, PBC_DT, G.CL2, box, steps, prec)
+            elif prec == "f64":
+                ctx.force_soa_f64(dq, dp, PBC_DT, G.CL2, box, steps)
+            else:
+                ctx.force_soa_mixed(dq, dp, PBC_DT, G.CL2, box, steps, prec)
+        ctx.sync()
+        if layout == "aos":
+            np.testing.assert_array_equal(dq.download(), q)
+            res.append(dp.download())
+        else:
+            res.append(np.stack([d.download() for d in dp], 1))
+    ctx.set_graphs(True)
+    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    np.testing.assert_array_equal(res[0], res[1])
+    if prec == "f64":
+        assert normwise(res[0][:, :3], oracle_momenta(12, 1.0, steps=20)) < FP64_TOL

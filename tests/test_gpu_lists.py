@@ -6,13 +6,14 @@ list; the CSR it installs must be bit-identical to the host builder's
 makepair (tests/test_host_lists.py) and to the compiled reference's CSR
 (tests/test_oracle_golden.py). Layouts derived on the device from a CSR (the
 full list of a half list, row ordering, LDS tiles) must equal their host
-restatements (expand_half_to_full via makepair FULL, ljgpu_make_tiles).
+restatements (expand_half_to_full via makepair FULL, ljgpu_make_tiles), and
+the spatial slot order of the tiles the numpy restatement in helpers.py.
 """
 import numpy as np
 import pytest
 
 import lj_gpu_amd as G
-from helpers import FP64_TOL, PBC_DT, normwise, oracle_momenta, periodic
+from helpers import FP64_TOL, PBC_DT, column_order, normwise, oracle_momenta, periodic
 
 pytestmark = pytest.mark.gpu
 
@@ -129,10 +130,12 @@ def test_device_tiles_match_host_tiles(ctx, k, rho, lanes, tile):
     ctx.set_kernel_config(lanes, 256, 2)
     ctx.set_tiling(tile)
     ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_tile_order(G.TILE_ORDER_INDEX)
     ctx.set_list(full)
     t = ctx.get_tiles()
-    assert t is not None
     info = ctx.list_info()
+    ctx.set_tile_order(G.TILE_ORDER_SPATIAL)
+    assert t is not None
     h = G.make_tiles(full, info.tile_particles)
     # Identical whenever the host tiler did not have to shrink a block.
     if np.all(h.desc[:-1, 1] == info.tile_particles):
@@ -185,3 +188,86 @@ def test_device_build_large_system(ctx):
     host = G.makepair(q, box, G.LIST_FULL)
     got, _ = _device_build(ctx, q, box, G.LIST_FULL)
     _same(got, host)
+
+
+def _check_spatial_tiles(ctx, q, box, full, tile):
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_SPATIAL
+    np.testing.assert_array_equal(member, column_order(q, box, tile))
+    t = ctx.get_tiles()
+    h = G.make_tiles(full, tile, member)
+    assert t.max_stage == ctx.list_info().tile_max_stage
+    # Identical whenever the host tiler did not have to shrink a block.
+    if np.all(h.desc[:-1, 1] == tile):
+        np.testing.assert_array_equal(t.desc, h.desc)
+        np.testing.assert_array_equal(t.stage, h.stage)
+        np.testing.assert_array_equal(t.local, h.local)
+    return t
+
+
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 0.5)])
+def test_spatial_tiles_of_a_host_list(ctx, k, rho):
+    """A host list gets index-order tiles at installation and the spatial
+    order at its first force evaluation (which brings the positions)."""
+    q, box, half, full = periodic(k, rho)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_INDEX
+    np.testing.assert_array_equal(member, np.arange(full.n))
+    dq, dp = ctx.array(q), ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, 1)
+    ctx.sync()
+    t = _check_spatial_tiles(ctx, q, box, full, 64)
+    if k == 31:
+        assert t.max_stage < 0.8 * G.make_tiles(full, 64).max_stage
+
+
+def test_spatial_tiles_of_a_device_list(ctx):
+    """ljgpu_build_list orders the tiles from its own positions at once; the
+    half list expanded on the device tiles the same way."""
+    q, box, half, full = periodic(12, 1.0)
+    ctx.set_kernel_config(8, 256, 2)
+    ctx.set_tiling(32)
+    for kind in (G.LIST_FULL, G.LIST_HALF):
+        _device_build(ctx, q, box, kind)
+        _check_spatial_tiles(ctx, q, box, full, 32)
+
+
+def test_open_box_keeps_index_order(ctx):
+    q = G.lattice_reference(0.5, 20.0)
+    full = G.makepair(q, 0.0, G.LIST_FULL)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_list(full)
+    dq, dp = ctx.array(q), ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, 1.0, G.CL2, 0.0, 1)
+    ctx.sync()
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_INDEX
+    np.testing.assert_array_equal(member, np.arange(full.n))
+
+
+def test_tile_order_switch_and_untiled(ctx):
+    q, box, half, full = periodic(8, 1.0)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    dq = ctx.array(q)
+    ctx.build_list(dq, q.shape[0], box, G.LIST_FULL)
+    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    ctx.set_tile_order(G.TILE_ORDER_INDEX)
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_INDEX
+    np.testing.assert_array_equal(member, np.arange(full.n))
+    ctx.set_tile_order(G.TILE_ORDER_SPATIAL)
+    dp = ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, 1)
+    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    with pytest.raises(G.LJGPUError) as e:
+        ctx.set_tile_order(2)
+    assert e.value.code == G.ERR_INVALID_ARGUMENT
+    ctx.set_tiling(0)
+    assert ctx.get_tile_order() == (-1, None)
+    ctx.set_tiling(64)

@@ -7,7 +7,7 @@ import pytest
 
 import lj_gpu_amd as G
 import oracle as O
-from helpers import FP64_TOL, PBC_DT, normwise
+from helpers import FP64_TOL, PBC_DT, column_order, normwise
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +56,23 @@ def test_batched_build_matches_host_lists(ctx, work, kind):
     np.testing.assert_array_equal(got.pointer, ref.pointer)
     np.testing.assert_array_equal(got.sorted_list, ref.sorted_list)
     ctx.set_half_mode(G.HALF_AUTO)
+
+
+def test_batched_spatial_tile_order(ctx, work):
+    """Each replica's particles are ordered within its own slot range (the
+    numpy restatement over all replicas), so no tile mixes replicas."""
+    q, off, boxes = work
+    configure(ctx)
+    ctx.set_replicas(off, boxes)
+    ctx.build_list(ctx.array(q), q.shape[0], 0.0, G.LIST_FULL)
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_SPATIAL
+    np.testing.assert_array_equal(member, column_order(q, 0.0, 32, off, boxes))
+    t = ctx.get_tiles()
+    for i0, ni, *_rest in t.desc:
+        r = np.searchsorted(off, i0, side="right") - 1
+        assert off[r] <= member[i0: i0 + ni].min() and member[i0: i0 + ni].max() < off[r + 1]
+    ctx.set_replicas([], [])
 
 
 def batched_momenta(ctx, q, off, boxes, steps=100, mixed=None, build=True, graphs=True):
