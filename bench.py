@@ -248,8 +248,11 @@ def roofline(bytes_per_launch: int, seconds: float, traffic=None) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 5000; 1000 for the md workload)")
+    # At least one graph chunk (100 launches), so that no capture falls in
+    # the timed region, and long enough for the clocks to settle.
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--cells", type=int, default=31)
     ap.add_argument("--density", type=float, default=1.0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: library default)")
@@ -274,6 +277,8 @@ def main() -> int:
                     help="after timing, run three device-to-device copies of this many MiB "
                          "(a known-size kernel that calibrates rocprofv3's byte counters)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 1000 if args.workload == "md" else 5000
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

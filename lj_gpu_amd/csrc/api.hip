@@ -285,10 +285,13 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
       lv->tile_desc = ctx->tiles.desc.as<int32_t>();
       lv->tile_stage = ctx->tiles.stage.as<int32_t>();
       lv->tile_local = ctx->tiles.local.as<uint16_t>();
-      lv->tile_member = ctx->tiles.member.as<int32_t>();
-      lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
-      if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL && ctx->tiles.staged > 0)
-        lv->tile_stage_slot = ctx->tiles.stage_slot.as<int32_t>();
+      if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL) {
+        lv->tile_member = ctx->tiles.member.as<int32_t>();
+        lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
+        if (ctx->tiles.staged > 0) lv->tile_stage_slot = ctx->tiles.stage_slot.as<int32_t>();
+      } else {
+        lv->tile_slot_ptr = ctx->eval.ptr.as<int32_t>();  // slot = particle
+      }
     }
   }
   return LJGPU_OK;
@@ -764,6 +767,10 @@ int ljgpu_get_tile_order(ljgpu_ctx* ctx, int* order, int32_t* member, int64_t ca
   *order = tiled ? t.order : -1;
   if (!tiled || !member) return LJGPU_OK;
   if (capacity < ctx->eval.n) return set_error(LJGPU_ERR_CAPACITY, "get_tile_order: output too small");
+  if (t.order == LJGPU_TILE_ORDER_INDEX) {
+    for (int32_t s = 0; s < ctx->eval.n; s++) member[s] = s;
+    return LJGPU_OK;
+  }
   HIP_TRY(hipMemcpyAsync(member, t.member.p, sizeof(int32_t) * ctx->eval.n, hipMemcpyDeviceToHost,
                          ctx->stream), "hipMemcpyAsync");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

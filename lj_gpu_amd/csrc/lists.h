@@ -40,7 +40,9 @@ void dev_release(DevCSR& c);
 
 // LDS tiling of a row-ordered full list (the layout ListView describes).
 // Blocks cover consecutive slots of a particle order: slot s holds particle
-// member[s], whose row occupies local[slot_ptr[s] .. slot_ptr[s+1]).
+// member[s], whose row occupies local[slot_ptr[s] .. slot_ptr[s+1]). In
+// index order (member[s] = s) neither array is kept: the list's own
+// pointer serves as slot_ptr.
 struct DevTiles {
   int32_t tile_particles = 0;  // particles per block (0: no tiling)
   int32_t blocks = 0;

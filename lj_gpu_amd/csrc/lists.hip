@@ -761,7 +761,7 @@ __device__ int32_t tile_rows(const TileGeom& g, const int32_t* __restrict__ memb
                              const int32_t* __restrict__ ptr, const int32_t* __restrict__ slot_ptr,
                              TileRows& r) {
   for (int32_t t = threadIdx.x; t < g.ni; t += kBlock) {
-    const int32_t i = member[g.i0 + t];
+    const int32_t i = member ? member[g.i0 + t] : g.i0 + t;
     r.mem[t] = i;
     r.src[t] = ptr[i];
     r.beg[t] = slot_ptr[g.i0 + t] - static_cast<int32_t>(g.e0);
@@ -975,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
     // (spatial order only) their slots.
     for (int32_t t = threadIdx.x; t < ns; t += kBlock) {
       const int32_t key = uniq[t];
-      stage[s0 + t] = slot_of ? member[key] : key;
+      stage[s0 + t] = member ? member[key] : key;
       if (stage_slot) stage_slot[s0 + t] = key;
       rank_of[tile_find(table, key)] = static_cast<uint16_t>(t);
     }
@@ -1099,12 +1099,6 @@ __global__ void k_order_sort(int64_t nkey, const int32_t* __restrict__ start,
       order[m] = v;
     }
   }
-}
-
-__global__ void k_iota(int32_t n, int32_t* __restrict__ a) {
-  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
-       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    a[t] = static_cast<int32_t>(t);
 }
 
 // slot_of[member[s]] = s.
@@ -1562,33 +1556,32 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
   out.blocks = 0;
   if (T <= 0 || T > kTileRowsMax || c.n <= 0) return LJGPU_OK;
   // The slot order and each slot's row start in the slot-ordered rows.
-  LCHECK(dev_reserve(out.member, sizeof(int32_t) * c.n, s));
-  LCHECK(dev_reserve(out.slot_ptr, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
-  int32_t* mem = out.member.as<int32_t>();
-  if (member) {
-    LTRY(hipMemcpyAsync(mem, member, sizeof(int32_t) * c.n, hipMemcpyDeviceToDevice, s),
-         "hipMemcpyAsync");
-  } else {
-    k_iota<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem);
-    LTRY(hipGetLastError(), "iota kernel");
-  }
+  // Index order: slot = particle and the list's own row starts. Otherwise
+  // the slot order, the slot of each particle (blocks key their particles
+  // by slot) and each slot's row start in the rows stored in slot order.
   out.order = member ? LJGPU_TILE_ORDER_SPATIAL : LJGPU_TILE_ORDER_INDEX;
-  // Blocks key their particles by slot: slot_of[member[s]] = s.
+  const int32_t* mem = nullptr;
   const int32_t* slot_of = nullptr;
+  const int32_t* sptr = c.ptr.as<int32_t>();
   if (member) {
+    LCHECK(dev_reserve(out.member, sizeof(int32_t) * c.n, s));
+    LCHECK(dev_reserve(out.slot_ptr, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
     LCHECK(dev_reserve(w.slot_of, sizeof(int32_t) * c.n, s));
-    k_inverse<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem, w.slot_of.as<int32_t>());
-    LTRY(hipGetLastError(), "inverse order kernel");
+    LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
+    int32_t* m = out.member.as<int32_t>();
+    LTRY(hipMemcpyAsync(m, member, sizeof(int32_t) * c.n, hipMemcpyDeviceToDevice, s),
+         "hipMemcpyAsync");
+    k_inverse<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, m, w.slot_of.as<int32_t>());
+    k_slot_nop<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, m, c.nop.as<int32_t>(),
+                                                         w.counts.as<int32_t>());
+    LTRY(hipGetLastError(), "tile order kernels");
+    int64_t e = 0;
+    LCHECK(dev_scan(s, w.counts.as<int32_t>(), out.slot_ptr.as<int32_t>(), c.n, w, &e));
+    if (e != c.e) return set_error(LJGPU_ERR_HIP, "tile order is not a permutation of the rows");
+    mem = m;
     slot_of = w.slot_of.as<int32_t>();
+    sptr = out.slot_ptr.as<int32_t>();
   }
-  LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(c.n) + 1), s));
-  k_slot_nop<<<grid_for(c.n, kBlock), kBlock, 0, s>>>(c.n, mem, c.nop.as<int32_t>(),
-                                                       w.counts.as<int32_t>());
-  LTRY(hipGetLastError(), "slot row length kernel");
-  int64_t e = 0;
-  LCHECK(dev_scan(s, w.counts.as<int32_t>(), out.slot_ptr.as<int32_t>(), c.n, w, &e));
-  if (e != c.e) return set_error(LJGPU_ERR_HIP, "tile order is not a permutation of the rows");
-  const int32_t* sptr = out.slot_ptr.as<int32_t>();
   // Blocks of T consecutive slots, restarted at every replica boundary.
   std::vector<int32_t> geo;
   const std::vector<int32_t> single = {0, c.n};

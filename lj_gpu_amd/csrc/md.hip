@@ -225,6 +225,21 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     return set_error(LJGPU_ERR_UNSUPPORTED, "molecular dynamics of batched replicas");
   std::memset(stats, 0, sizeof(*stats));
   TRY(use_device(ctx));
+  // Tiles in index order: the spatial order costs more per rebuild (about
+  // 11 ms at 8.4 M particles and 1.5e9 entries) than it saves over the
+  // kicks between rebuilds. The caller's choice is restored on return.
+  struct OrderGuard {
+    ljgpu_ctx* c;
+    int saved;
+    ~OrderGuard() {
+      if (c->tile_order != saved) {
+        c->tile_order = saved;
+        c->tiles_for = -1;
+        c->tiles_ordered = false;
+      }
+    }
+  } order_guard{ctx, ctx->tile_order};
+  ctx->tile_order = LJGPU_TILE_ORDER_INDEX;
   const double dt = cfg->dt, cl2 = cfg->cutoff * cfg->cutoff;
   const double search = cfg->cutoff + cfg->skin;
   const bool mixed = cfg->precision == LJGPU_PREC_MIXED;
