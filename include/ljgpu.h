@@ -342,7 +342,11 @@ typedef struct ljgpu_md_stats {
  * sampled at step 0, every sample_every steps and at the end. `samples`
  * (optional, host) receives up to sample_capacity rows (step, U, K, U + K)
  * with U the shifted potential (see ljgpu_md_stats).
- * The context's list is replaced by the run's list. */
+ * In a periodic box the run works on copies of q and p in a spatial order of
+ * the particles, re-sorted at every rebuild, so that index-order tiles over
+ * them are compact. It writes them back to q and p in the caller's order on
+ * return. Its lists index that order, so none stays installed. With
+ * box <= 0 the context's list is replaced by the run's last list. */
 int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t particle_number, double box,
                  const ljgpu_md_config* config, int32_t steps, ljgpu_md_stats* stats,
                  double* samples, int32_t sample_capacity);

@@ -47,6 +47,8 @@ void release_all(ljgpu_ctx* ctx) {
   dev_release(ctx->work);
   dev_release(ctx->slot_q);
   dev_release(ctx->slot_p);
+  for (DevMem* m : {&ctx->md_q, &ctx->md_p, &ctx->md_tmp, &ctx->md_perm, &ctx->md_perm2})
+    dev_release(*m);
 }
 
 // Particles per tile for the current kernel configuration: the requested

@@ -91,6 +91,9 @@ struct ljgpu_ctx {
   // scratch.
   ljgpu::DevMem md_qref;
   ljgpu::DevMem md_red;
+  // The loop's working copies of q and p in a spatial order, a staging
+  // buffer, and the caller's index of each slot (two buffers).
+  ljgpu::DevMem md_q, md_p, md_tmp, md_perm, md_perm2;
 
   int lanes = 4;
   int unroll = 4;
@@ -117,6 +120,10 @@ inline int hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 int use_device(ljgpu_ctx* ctx);
+// Forget the installed list (and its layouts).
+void reset_list(ljgpu_ctx* ctx);
+// Particles per tile for the current kernel configuration (0: untiled).
+int32_t effective_tile(const ljgpu_ctx* ctx);
 // Derive the layouts the configured kernel needs. Positions (q, with the
 // periodic box, 0 for none) let the tiles take the spatial order.
 int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q = nullptr, double box = 0.0);

@@ -145,6 +145,35 @@ __global__ __launch_bounds__(kBlock) void k_kinetic(int32_t n, const double* __r
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// Working copies in slot order: out[s] = in[order[s]] (gather) or
+// out[order[s]] = in[s] (scatter), for 4-double rows and int32 values.
+__global__ void k_gather_rows(int32_t n, const int32_t* __restrict__ order,
+                              const double4* __restrict__ in, double4* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[t] = in[order[t]];
+}
+
+__global__ void k_scatter_rows(int32_t n, const int32_t* __restrict__ order,
+                               const double4* __restrict__ in, double4* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[order[t]] = in[t];
+}
+
+__global__ void k_gather_i32(int32_t n, const int32_t* __restrict__ order,
+                             const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[t] = in[order[t]];
+}
+
+__global__ void k_iota_i32(int32_t n, int32_t* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[t] = static_cast<int32_t>(t);
+}
+
 int grid_for(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65535)));
 }
@@ -225,9 +254,10 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     return set_error(LJGPU_ERR_UNSUPPORTED, "molecular dynamics of batched replicas");
   std::memset(stats, 0, sizeof(*stats));
   TRY(use_device(ctx));
-  // Tiles in index order: the spatial order costs more per rebuild (about
-  // 11 ms at 8.4 M particles and 1.5e9 entries) than it saves over the
-  // kicks between rebuilds. The caller's choice is restored on return.
+  // Tiles in index order over slot-ordered working copies (below): the
+  // copies are already in a spatial order, so consecutive slots are compact
+  // tiles without the per-entry slot lookups of spatially ordered tiles over
+  // the caller's arrays. The caller's choice is restored on return.
   struct OrderGuard {
     ljgpu_ctx* c;
     int saved;
@@ -244,11 +274,56 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   const double search = cfg->cutoff + cfg->skin;
   const bool mixed = cfg->precision == LJGPU_PREC_MIXED;
   const int accum = cfg->accum;
+  // In a periodic box the loop runs on copies of q and p in a spatial
+  // order of the particles, re-sorted at every list rebuild
+  // (dev_tile_order): slot s holds the caller's particle perm[s]. The
+  // copies go back to the caller's order on return, and the list, which
+  // indexes slots, is uninstalled then. An open box runs on q and p.
+  const bool slots = box > 0.0 && n > 0;
+  double* wq = q;
+  double* wp = p;
+  int32_t* perm = nullptr;
+  int32_t* perm_spare = nullptr;
+  const size_t row_bytes = sizeof(double) * 4 * static_cast<size_t>(std::max(n, 1));
+  const int grid_n = grid_for(n);
+  if (slots) {
+    for (DevMem* m : {&ctx->md_q, &ctx->md_p, &ctx->md_tmp})
+      TRY(dev_reserve(*m, row_bytes, ctx->stream));
+    for (DevMem* m : {&ctx->md_perm, &ctx->md_perm2})
+      TRY(dev_reserve(*m, sizeof(int32_t) * n, ctx->stream));
+    wq = ctx->md_q.as<double>();
+    wp = ctx->md_p.as<double>();
+    perm = ctx->md_perm.as<int32_t>();
+    perm_spare = ctx->md_perm2.as<int32_t>();
+    HIP_TRY(hipMemcpyAsync(wq, q, row_bytes, hipMemcpyDeviceToDevice, ctx->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(wp, p, row_bytes, hipMemcpyDeviceToDevice, ctx->stream),
+            "hipMemcpyAsync");
+    k_iota_i32<<<grid_n, kBlock, 0, ctx->stream>>>(n, perm);
+    HIP_TRY(hipGetLastError(), "slot index kernel");
+  }
+  bool returned = !slots;
+  auto give_back = [&]() -> int {
+    if (returned) return LJGPU_OK;
+    returned = true;
+    k_scatter_rows<<<grid_n, kBlock, 0, ctx->stream>>>(n, perm, reinterpret_cast<double4*>(wq),
+                                                       reinterpret_cast<double4*>(q));
+    k_scatter_rows<<<grid_n, kBlock, 0, ctx->stream>>>(n, perm, reinterpret_cast<double4*>(wp),
+                                                       reinterpret_cast<double4*>(p));
+    HIP_TRY(hipGetLastError(), "slot scatter kernels");
+    reset_list(ctx);
+    return LJGPU_OK;
+  };
+  struct GiveBack {
+    std::function<int()> f;
+    ~GiveBack() { (void)f(); }
+  } give_back_guard{give_back};
   PosIn in;
-  in.x = q;
-  in.y = q + 1;
-  in.z = q + 2;
+  in.x = wq;
+  in.y = wq + 1;
+  in.z = wq + 2;
   in.stride = 4;
+  const int32_t order_tile = effective_tile(ctx) > 0 ? effective_tile(ctx) : 64;
   hipEvent_t e0 = nullptr, e1 = nullptr, b0 = nullptr, b1 = nullptr;
   auto cleanup = [&]() {
     for (hipEvent_t e : {e0, e1, b0, b1})
@@ -270,12 +345,30 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   TRY(dev_reserve(ctx->md_qref, sizeof(double) * 4 * std::max(n, 1), ctx->stream));
   auto rebuild = [&]() -> int {
     HIP_TRY(hipEventRecord(b0, ctx->stream), "hipEventRecord");
+    if (slots) {
+      // Re-sort the working copies: new slot s takes old slot order[s].
+      bool ordered = false;
+      TRY(dev_tile_order(ctx->stream, in, n, box, {}, {}, order_tile, ctx->work, &ordered));
+      if (ordered) {
+        const int32_t* order = ctx->work.order.as<int32_t>();
+        double4* tmp = ctx->md_tmp.as<double4>();
+        for (double* w : {wq, wp}) {
+          k_gather_rows<<<grid_n, kBlock, 0, ctx->stream>>>(n, order,
+                                                            reinterpret_cast<double4*>(w), tmp);
+          HIP_TRY(hipMemcpyAsync(w, tmp, row_bytes, hipMemcpyDeviceToDevice, ctx->stream),
+                  "hipMemcpyAsync");
+        }
+        k_gather_i32<<<grid_n, kBlock, 0, ctx->stream>>>(n, order, perm, perm_spare);
+        HIP_TRY(hipGetLastError(), "slot gather kernels");
+        std::swap(perm, perm_spare);
+      }
+    }
     // Rows in the build's own order: the dynamics need the pair set, not
     // the reference's row order, and that order is fixed by the positions.
     TRY(build_list(ctx, in, n, box, search, LJGPU_LIST_FULL, false));
     if (n > 0)
-      HIP_TRY(hipMemcpyAsync(ctx->md_qref.p, q, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice,
-                             ctx->stream), "hipMemcpyAsync");
+      HIP_TRY(hipMemcpyAsync(ctx->md_qref.p, wq, sizeof(double) * 4 * n,
+                             hipMemcpyDeviceToDevice, ctx->stream), "hipMemcpyAsync");
     HIP_TRY(hipEventRecord(b1, ctx->stream), "hipEventRecord");
     HIP_TRY(hipEventSynchronize(b1), "hipEventSynchronize");
     float ms = 0.0f;
@@ -293,7 +386,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
         reinterpret_cast<unsigned long long*>(ctx->md_red.as<double>() + 3 * kEnergyBlocks);
     HIP_TRY(hipMemsetAsync(slot, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
     k_displacement<<<std::min(1024, grid_for(n)), kBlock, 0, ctx->stream>>>(
-        n, q, ctx->md_qref.as<double>(), box, slot);
+        n, wq, ctx->md_qref.as<double>(), box, slot);
     HIP_TRY(hipGetLastError(), "displacement kernel");
     unsigned long long bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream),
@@ -306,8 +399,8 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   TRY(rebuild());
   ForceArgs fa;
   fa.soa = false;
-  fa.q[0] = q;
-  fa.p[0] = p;
+  fa.q[0] = wq;
+  fa.p[0] = wp;
   fa.cl2 = cl2;
   fa.box = box;
   int32_t nsamples = 0;
@@ -319,7 +412,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   const double u_cut = 4.0 * s6 * (s6 - 1.0);
   auto sample = [&](int32_t step) -> int {
     double u = 0.0, k = 0.0, pairs = 0.0;
-    TRY(energy(ctx, q, p, box, cl2, &u, &k, &pairs));
+    TRY(energy(ctx, wq, wp, box, cl2, &u, &k, &pairs));
     u -= pairs * u_cut;
     if (nsamples == 0) {
       stats->potential0 = u;
@@ -346,7 +439,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   };
   auto drift = [&]() -> int {
     if (n == 0) return LJGPU_OK;
-    k_drift<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, q, p, dt, box);
+    k_drift<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, wq, wp, dt, box);
     HIP_TRY(hipGetLastError(), "drift kernel");
     return LJGPU_OK;
   };
@@ -443,6 +536,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     s += len;
   }
 
+  TRY(give_back());
   HIP_TRY(hipEventRecord(e1, ctx->stream), "hipEventRecord");
   HIP_TRY(hipEventSynchronize(e1), "hipEventSynchronize");
   float total_ms = 0.0f;

@@ -282,3 +282,32 @@ def test_md_crowded_cell_falls_back_to_sorted_rows(ctx):
     full = G.makepair(q, box, G.LIST_FULL, search_length=3.3)
     assert out[0][3] == 2  # the first build and the one at step 1
     assert out[0][2] == 2 * full.n_entries  # two kicks over the same list
+
+
+def test_md_slot_order_copies_return_to_caller_order(ctx):
+    """In a periodic box the loop runs on spatially re-sorted copies of q and
+    p and writes them back in the caller's order: a run with a rebuild at
+    every step equals, particle by particle, one with a single build (the
+    same dynamics, different slot orders), and no list stays installed."""
+    q0, p0, box = liquid(6)
+    out = []
+    for every in (1, 1000):
+        dq, dp = ctx.array(q0), ctx.array(p0)
+        ctx.md_run(dq, dp, box, 5, DT, skin=0.3, rebuild_every=every)
+        out.append((dq.download(), dp.download()))
+    d = out[0][0][:, :3] - out[1][0][:, :3]
+    d -= box * np.rint(d / box)
+    assert np.abs(d).max() < 1e-12
+    assert np.abs(out[0][1][:, :3] - out[1][1][:, :3]).max() < 1e-10
+    # particles moved by about |p| dt per step: the caller's order is kept
+    dq0 = out[0][0][:, :3] - q0[:, :3]
+    dq0 -= box * np.rint(dq0 / box)
+    assert np.abs(dq0).max() < 0.2
+    with pytest.raises(G.LJGPUError) as e:
+        ctx.list_info()
+    assert e.value.code == G.ERR_NO_LIST
+    # the caller's tile order is restored
+    ctx.set_list(G.makepair(q0, box, G.LIST_FULL))
+    dq, dp = ctx.array(q0), ctx.zeros(q0.shape)
+    ctx.force_aos_f64(dq, dp, DT, G.CL2, box, 1)
+    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
