@@ -31,13 +31,26 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kEnergyBlocks = 1024;  // fixed grid: partial sums reduce in a fixed order
 
-__device__ __forceinline__ double wrap(double x, double box) {
-  x -= box * floor(x / box);
-  return x >= box ? x - box : x;  // rounding can land exactly on box
+// x mapped into [0, box): one multiply by 1/box instead of a divide, then
+// the last-bit cases (a result just below 0, or equal to box) folded in.
+__device__ __forceinline__ double wrap(double x, double box, double inv_box) {
+  x = fma(-box, floor(x * inv_box), x);
+  if (x < 0.0) x += box;
+  return x >= box ? x - box : x;
 }
 
+__device__ __forceinline__ double image(double d, double box, double inv_box) {
+  return box > 0.0 ? fma(-box, rint(d * inv_box), d) : d;
+}
+
+// Drift q += p dt (wrapped into [0, box) for box > 0). With qref, also the
+// largest squared displacement |q - qref|^2 (minimum image) since the last
+// build, as the bit pattern of a non-negative double (integer order = numeric
+// order) maximised into *out.
 __global__ void k_drift(int32_t n, double* __restrict__ q, const double* __restrict__ p, double dt,
-                        double box) {
+                        double box, double inv_box, const double* __restrict__ qref,
+                        unsigned long long* __restrict__ out) {
+  double m = 0.0;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     double2 xy = reinterpret_cast<double2*>(q + 4 * i)[0];
@@ -47,34 +60,25 @@ __global__ void k_drift(int32_t n, double* __restrict__ q, const double* __restr
     xy.y += pxy.y * dt;
     z += p[4 * i + 2] * dt;
     if (box > 0.0) {
-      xy.x = wrap(xy.x, box);
-      xy.y = wrap(xy.y, box);
-      z = wrap(z, box);
+      xy.x = wrap(xy.x, box, inv_box);
+      xy.y = wrap(xy.y, box, inv_box);
+      z = wrap(z, box, inv_box);
     }
     reinterpret_cast<double2*>(q + 4 * i)[0] = xy;
     q[4 * i + 2] = z;
+    if (qref) {
+      const double2 rxy = reinterpret_cast<const double2*>(qref + 4 * i)[0];
+      const double dx = image(xy.x - rxy.x, box, inv_box);
+      const double dy = image(xy.y - rxy.y, box, inv_box);
+      const double dz = image(z - qref[4 * i + 2], box, inv_box);
+      m = fmax(m, dx * dx + dy * dy + dz * dz);
+    }
   }
-}
-
-__device__ __forceinline__ double image(double d, double box) {
-  return box > 0.0 ? d - box * rint(d / box) : d;
-}
-
-// Largest displacement |q - qref| (minimum image), as the bit pattern of a
-// non-negative double, whose integer order is the numeric order.
-__global__ void k_displacement(int32_t n, const double* __restrict__ q,
-                               const double* __restrict__ qref, double box,
-                               unsigned long long* __restrict__ out) {
-  double m = 0.0;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const double dx = image(q[4 * i] - qref[4 * i], box);
-    const double dy = image(q[4 * i + 1] - qref[4 * i + 1], box);
-    const double dz = image(q[4 * i + 2] - qref[4 * i + 2], box);
-    m = fmax(m, sqrt(dx * dx + dy * dy + dz * dz));
+  if (qref) {
+    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0)
+      atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
   }
-  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
 }
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -91,14 +95,16 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // (the force kernels' cutoff rule), and the number of those pairs, times
 // `weight` (1/2 for a full list, which holds every pair twice). Eight lanes
 // share a row (lane k takes entries k, k + 8, ...), so index loads are
-// contiguous; one partial sum of each quantity per workgroup.
+// contiguous; each lane loads kEnergyBatch partners before using any. One
+// partial sum of each quantity per workgroup.
 constexpr int kEnergyLanes = 8;
+constexpr int kEnergyBatch = 4;
 
 __global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* __restrict__ nop,
                                                       const int32_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ list,
                                                       const double* __restrict__ q, double cl2,
-                                                      double box, double weight,
+                                                      double box, double inv_box, double weight,
                                                       double* __restrict__ partial,
                                                       double* __restrict__ partial_count) {
   __shared__ double sh[kBlock / 64];
@@ -111,17 +117,32 @@ __global__ __launch_bounds__(kBlock) void k_potential(int32_t n, const int32_t* 
     const double xi = q[4 * i], yi = q[4 * i + 1], zi = q[4 * i + 2];
     const int64_t kp = ptr[i];
     const int32_t np = nop[i];
-    for (int32_t k = sub; k < np; k += kEnergyLanes) {
-      const int64_t j = list[kp + k];
-      const double dx = image(q[4 * j] - xi, box);
-      const double dy = image(q[4 * j + 1] - yi, box);
-      const double dz = image(q[4 * j + 2] - zi, box);
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 > cl2) continue;
-      const double s = 1.0 / r2;
-      const double s3 = s * s * s;
-      u += 4.0 * s3 * (s3 - 1.0);
-      cnt += 1.0;
+    for (int32_t k0 = sub; k0 < np; k0 += kEnergyLanes * kEnergyBatch) {
+      int64_t j[kEnergyBatch];
+#pragma unroll
+      for (int b = 0; b < kEnergyBatch; b++) {
+        const int32_t k = k0 + b * kEnergyLanes;
+        j[b] = k < np ? list[kp + k] : i;  // past the row: the particle itself (r2 = 0)
+      }
+      double x[kEnergyBatch], y[kEnergyBatch], z[kEnergyBatch];
+#pragma unroll
+      for (int b = 0; b < kEnergyBatch; b++) {
+        x[b] = q[4 * j[b]];
+        y[b] = q[4 * j[b] + 1];
+        z[b] = q[4 * j[b] + 2];
+      }
+#pragma unroll
+      for (int b = 0; b < kEnergyBatch; b++) {
+        const double dx = image(x[b] - xi, box, inv_box);
+        const double dy = image(y[b] - yi, box, inv_box);
+        const double dz = image(z[b] - zi, box, inv_box);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (k0 + b * kEnergyLanes >= np || r2 > cl2) continue;
+        const double s = 1.0 / r2;
+        const double s3 = s * s * s;
+        u += 4.0 * s3 * (s3 - 1.0);
+        cnt += 1.0;
+      }
     }
   }
   const double t = block_sum(u * weight, sh);
@@ -194,7 +215,8 @@ int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double 
     k_potential<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, ctx->eval.nop.as<int32_t>(),
                                                   ctx->eval.ptr.as<int32_t>(),
                                                   ctx->eval.list.as<int32_t>(), q, cl2, box,
-                                                  weight, part, part + 2 * kEnergyBlocks);
+                                                  box > 0.0 ? 1.0 / box : 0.0, weight, part,
+                                                  part + 2 * kEnergyBlocks);
   }
   if (p) k_kinetic<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, p, part + kEnergyBlocks);
   HIP_TRY(hipGetLastError(), "energy kernels");
@@ -377,25 +399,6 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     rebuilds++;
     return LJGPU_OK;
   };
-  // Largest displacement since the last build (synchronises).
-  auto displacement = [&](double* out) -> int {
-    *out = 0.0;
-    if (n == 0) return LJGPU_OK;
-    TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
-    unsigned long long* slot =
-        reinterpret_cast<unsigned long long*>(ctx->md_red.as<double>() + 3 * kEnergyBlocks);
-    HIP_TRY(hipMemsetAsync(slot, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
-    k_displacement<<<std::min(1024, grid_for(n)), kBlock, 0, ctx->stream>>>(
-        n, wq, ctx->md_qref.as<double>(), box, slot);
-    HIP_TRY(hipGetLastError(), "displacement kernel");
-    unsigned long long bits = 0;
-    HIP_TRY(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream),
-            "hipMemcpyAsync");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    std::memcpy(out, &bits, sizeof(double));
-    return LJGPU_OK;
-  };
-
   TRY(rebuild());
   ForceArgs fa;
   fa.soa = false;
@@ -437,10 +440,30 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     stats->pair_entries += ctx->eval.e;
     return LJGPU_OK;
   };
-  auto drift = [&]() -> int {
+  const double inv_box = box > 0.0 ? 1.0 / box : 0.0;
+  // One drift; with `disp`, also the largest displacement since the last
+  // build (synchronises).
+  auto drift = [&](double* disp = nullptr) -> int {
+    if (disp) *disp = 0.0;
     if (n == 0) return LJGPU_OK;
-    k_drift<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, wq, wp, dt, box);
+    unsigned long long* slot = nullptr;
+    if (disp) {
+      TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
+      slot = reinterpret_cast<unsigned long long*>(ctx->md_red.as<double>() + 3 * kEnergyBlocks);
+      HIP_TRY(hipMemsetAsync(slot, 0, sizeof(unsigned long long), ctx->stream), "hipMemsetAsync");
+    }
+    k_drift<<<disp ? std::min(1024, grid_for(n)) : grid_for(n), kBlock, 0, ctx->stream>>>(
+        n, wq, wp, dt, box, inv_box, disp ? ctx->md_qref.as<double>() : nullptr, slot);
     HIP_TRY(hipGetLastError(), "drift kernel");
+    if (disp) {
+      unsigned long long bits = 0;
+      HIP_TRY(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream),
+              "hipMemcpyAsync");
+      HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+      double d2 = 0.0;
+      std::memcpy(&d2, &bits, sizeof(double));
+      *disp = std::sqrt(d2);
+    }
     return LJGPU_OK;
   };
 
@@ -467,18 +490,15 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
     const bool rebuild_now = !adaptive && s % cfg->rebuild_every == 0;
     const bool sample_now = (cfg->sample_every > 0 && s % cfg->sample_every == 0) || s == steps;
     if (adaptive || rebuild_now || sample_now || !ctx->use_graphs) {
-      TRY(drift());
+      double d = 0.0;
+      TRY(drift(adaptive || rebuild_now ? &d : nullptr));
       if (adaptive) {
-        double d = 0.0;
-        TRY(displacement(&d));
         if (2.0 * d > cfg->skin) {
           TRY(rebuild());
           d = 0.0;
         }
         max_disp = std::max(max_disp, d);
       } else if (rebuild_now) {
-        double d = 0.0;
-        TRY(displacement(&d));
         max_disp = std::max(max_disp, d);
         if (2.0 * d > cfg->skin) stale++;
         TRY(rebuild());
