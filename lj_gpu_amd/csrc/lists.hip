@@ -1503,6 +1503,12 @@ int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
       const double v = std::floor(L / side + 0.5);
       g.nc[c] = v >= 1024.0 ? 1024 : std::max(1, static_cast<int>(v));
     }
+    // A few particles in a large box: at most about 2 columns per particle,
+    // so the key arrays stay proportional to N.
+    while (static_cast<int64_t>(g.nc[1]) * g.nc[2] > 2 * static_cast<int64_t>(m) + 16) {
+      int& big = g.nc[1] >= g.nc[2] ? g.nc[1] : g.nc[2];
+      big = (big + 1) / 2;
+    }
     g.nc[0] = std::max<int32_t>(1, m / (g.nc[1] * g.nc[2]));
     for (int c = 0; c < 3; c++) g.w[c] = L / g.nc[c];
     g.key0 = static_cast<int32_t>(keys);

@@ -100,6 +100,9 @@ def column_order(q: np.ndarray, box: float, tile_particles: int = 64, offsets=No
         for c in (1, 2):
             v = np.floor(L / side + 0.5)
             nc[c] = 1024 if v >= 1024.0 else max(1, int(v))
+        while nc[1] * nc[2] > 2 * m + 16:  # at most about 2 columns per particle
+            big = 1 if nc[1] >= nc[2] else 2
+            nc[big] = (nc[big] + 1) // 2
         nc[0] = max(1, m // (nc[1] * nc[2]))
         cell = []
         for c in range(3):

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import lj_gpu_amd as G
+import oracle as O
 from helpers import FP64_TOL, PBC_DT, column_order, normwise, oracle_momenta, periodic
 
 pytestmark = pytest.mark.gpu
@@ -234,6 +235,48 @@ def test_spatial_tiles_of_a_device_list(ctx):
     for kind in (G.LIST_FULL, G.LIST_HALF):
         _device_build(ctx, q, box, kind)
         _check_spatial_tiles(ctx, q, box, full, 32)
+
+
+def _force_once(ctx, q, box, full):
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_list(full)
+    dq, dp = ctx.array(q), ctx.zeros(q.shape)
+    ctx.force_aos_f64(dq, dp, 1.0, G.CL2, box, 1)
+    ctx.sync()
+    return dp.download()
+
+
+def test_spatial_order_of_a_sparse_system_in_a_large_box(ctx):
+    """300 particles in a box of side 120: the column grid is capped at about
+    two columns per particle, and forces still match the oracle."""
+    rng = np.random.default_rng(8)
+    box = 120.0
+    q = np.zeros((300, 4))
+    q[:, :3] = box * rng.random((300, 3))
+    q[:150, :3] = 40.0 + 6.0 * rng.random((150, 3))  # a cluster with neighbours
+    full = G.makepair(q, box, G.LIST_FULL)
+    p = _force_once(ctx, q, box, full)
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_SPATIAL
+    np.testing.assert_array_equal(member, column_order(q, box, 64))
+    ref = np.zeros_like(q)
+    O.force_aos(O.CSR(full.number_of_partners, full.pointer, full.sorted_list, "full"),
+                q, ref, 1.0, O.CL2, box, method="full")
+    assert normwise(p, ref) < FP64_TOL
+
+
+def test_crowded_bin_keeps_index_order(ctx):
+    """More than 256 particles in one bin of the order: index order."""
+    rng = np.random.default_rng(9)
+    box = 20.0
+    q = np.zeros((600, 4))
+    q[:, :3] = box * rng.random((600, 3))
+    q[:400, :3] = 7.0 + 0.01 * rng.random((400, 3))
+    full = G.makepair(q, box, G.LIST_FULL)
+    _force_once(ctx, q, box, full)
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_INDEX
 
 
 def test_open_box_keeps_index_order(ctx):
