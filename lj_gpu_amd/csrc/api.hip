@@ -796,13 +796,30 @@ int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_
     return set_error(LJGPU_ERR_CAPACITY, "get_tiles: output too small");
   HIP_TRY(hipMemcpyAsync(desc, t.desc.p, sizeof(int32_t) * 8 * t.blocks, hipMemcpyDeviceToHost,
                          ctx->stream), "hipMemcpyAsync");
-  if (t.staged > 0)
-    HIP_TRY(hipMemcpyAsync(stage, t.stage.p, sizeof(int32_t) * t.staged, hipMemcpyDeviceToHost,
-                           ctx->stream), "hipMemcpyAsync");
   if (ctx->eval.e > 0)
     HIP_TRY(hipMemcpyAsync(local, t.local.p, sizeof(uint16_t) * ctx->eval.e, hipMemcpyDeviceToHost,
                            ctx->stream), "hipMemcpyAsync");
+  if (t.stage_stride == 0) {
+    if (t.staged > 0)
+      HIP_TRY(hipMemcpyAsync(stage, t.stage.p, sizeof(int32_t) * t.staged, hipMemcpyDeviceToHost,
+                             ctx->stream), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LJGPU_OK;
+  }
+  // One-pass build: each block's staging area sits at a fixed stride on the
+  // device; pack them back to back, as ljgpu_make_tiles lays them out.
+  std::vector<int32_t> sparse(static_cast<size_t>(t.blocks) * t.stage_stride);
+  HIP_TRY(hipMemcpyAsync(sparse.data(), t.stage.p, sizeof(int32_t) * sparse.size(),
+                         hipMemcpyDeviceToHost, ctx->stream), "hipMemcpyAsync");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  int64_t pos = 0;
+  for (int32_t b = 0; b < t.blocks; b++) {
+    const int32_t ns = desc[8 * b + 3];
+    std::memcpy(stage + pos, sparse.data() + static_cast<size_t>(b) * t.stage_stride,
+                sizeof(int32_t) * ns);
+    desc[8 * b + 2] = static_cast<int32_t>(pos);
+    pos += ns;
+  }
   return LJGPU_OK;
 }
 

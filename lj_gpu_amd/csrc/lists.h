@@ -50,6 +50,8 @@ struct DevTiles {
   int32_t max_stage = 0;
   int32_t lds_bytes = 0;
   int order = 0;               // LJGPU_TILE_ORDER_INDEX (member[s] = s) or _SPATIAL
+  int32_t stage_stride = 0;    // >0: block b stages at b * stage_stride (one-pass build);
+                               // 0: staging areas packed back to back
   DevMem desc, stage, local;
   DevMem member;      // int32 per slot
   DevMem slot_ptr;    // int32 per slot (+1)

@@ -717,6 +717,7 @@ constexpr int kTileHashLimit = 3072;                   // distinct keys before g
 constexpr int kTileMaxStage = kTileLdsBudget / 24 + 1; // staged positions that can fit
 constexpr int kTileUniq = 4096;                        // sort buffer (power of two)
 constexpr int kTileBatch = 8;                          // list entries loaded ahead per thread
+constexpr int64_t kTileOnePassBytes = int64_t{4} << 30;  // staging slots of the one-pass build
 static_assert(kTileMaxStage <= kTileHashLimit, "the hash limit must cover every tile that fits");
 static_assert(kTileHashLimit + kBlock < kTileHash, "probing must always find a free slot");
 static_assert(kTileHashLimit <= kTileUniq, "the sort buffer must hold every distinct key");
@@ -939,7 +940,9 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
                                                       int32_t* __restrict__ desc,
                                                       int32_t* __restrict__ stage,
                                                       int32_t* __restrict__ stage_slot,
-                                                      uint16_t* __restrict__ local) {
+                                                      uint16_t* __restrict__ local,
+                                                      int32_t* __restrict__ stats,
+                                                      unsigned long long* __restrict__ staged) {
   __shared__ int32_t table[kTileHash];
   __shared__ uint16_t rank_of[kTileHash];
   __shared__ int32_t uniq[kTileUniq];
@@ -951,6 +954,19 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
     const TileGeom g = tile_geom(b, geo, slot_ptr);
     const int32_t base = tile_rows(g, member, ptr, slot_ptr, rows);
     const int32_t ns = tile_insert(g, rows, base, list, slot_of, table, &counter);
+    if (!soff) {
+      // One pass: block b stages at b * kTileMaxStage, and the budget check
+      // of k_tile_count happens here (stats as there, plus the total staged).
+      const int64_t bytes = ns < 0 ? INT32_MAX : tile_lds_bytes(ns, g.ne);
+      const bool over = bytes > kTileLdsBudget;
+      if (threadIdx.x == 0) {
+        if (over) atomicOr(&stats[0], 1);
+        atomicMax(&stats[1], ns);
+        atomicMax(&stats[2], static_cast<int32_t>(std::min<int64_t>(bytes, INT32_MAX)));
+        if (!over) atomicAdd(staged, static_cast<unsigned long long>(ns));
+      }
+      if (over) continue;  // the whole block (ns is uniform); the caller retries smaller
+    }
     // Compact the occupied slots (C per thread) into uniq[0..ns), pad, sort.
     int32_t cnt = 0;
     for (int32_t t = threadIdx.x * C; t < (threadIdx.x + 1) * C; t++) cnt += table[t] != -1;
@@ -970,7 +986,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
       default: block_bitonic<16>(uniq); break;
     }
     __syncthreads();
-    const int64_t s0 = soff[b];
+    const int64_t s0 = soff ? soff[b] : static_cast<int64_t>(b) * kTileMaxStage;
     // Staged particles in slot order: stage holds the particles, stage_slot
     // (spatial order only) their slots.
     for (int32_t t = threadIdx.x; t < ns; t += kBlock) {
@@ -1601,37 +1617,65 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
   LTRY(hipMemcpyAsync(w.tile_geo.p, geo.data(), sizeof(int32_t) * geo.size(),
                       hipMemcpyHostToDevice, s), "hipMemcpyAsync");
   const int32_t* dgeo = w.tile_geo.as<int32_t>();
-  LCHECK(dev_reserve(w.tile_ns, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
-  LCHECK(dev_reserve(w.tile_soff, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
   LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
   int32_t* stats = w.small.as<int32_t>() + 4;
+  unsigned long long* staged_dev = w.small.as<unsigned long long>() + 4;
   LCHECK(fill<int32_t>(s, stats, 3, 0));
+  LCHECK(fill<unsigned long long>(s, staged_dev, 1, 0ull));
   const int grid = std::min(nb, 65535);
-  k_tile_count<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
-                                       c.list.as<int32_t>(), w.tile_ns.as<int32_t>(), stats);
-  LTRY(hipGetLastError(), "tile count kernel");
-  int32_t h[3];
-  LTRY(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
-  if (h[0] != 0) return LJGPU_OK;  // does not fit: caller retries smaller or runs untiled
-  int64_t staged = 0;
-  LCHECK(dev_scan(s, w.tile_ns.as<int32_t>(), w.tile_soff.as<int32_t>(), nb, w, &staged));
-  if (staged > INT32_MAX) return LJGPU_OK;
-  LCHECK(dev_reserve(out.desc, sizeof(int32_t) * 8 * static_cast<size_t>(nb), s));
-  LCHECK(dev_reserve(out.stage, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
   // The force kernel copies entry offsets in whole 16-byte chunks: pad.
   const int64_t nlocal = c.e + kTileLocalPad;
   LCHECK(dev_reserve(out.local, sizeof(uint16_t) * nlocal, s));
   LCHECK(fill<uint16_t>(s, out.local.as<uint16_t>() + c.e, kTileLocalPad, 0));
-  // The staged particles as slots too, for evaluations over copies of the
-  // positions and momenta in slot order.
-  if (member) LCHECK(dev_reserve(out.stage_slot, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
-  k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
-                                      c.list.as<int32_t>(), w.tile_soff.as<int32_t>(),
-                                      out.desc.as<int32_t>(), out.stage.as<int32_t>(),
-                                      member ? out.stage_slot.as<int32_t>() : nullptr,
-                                      out.local.as<uint16_t>());
-  LTRY(hipGetLastError(), "tile fill kernel");
+  LCHECK(dev_reserve(out.desc, sizeof(int32_t) * 8 * static_cast<size_t>(nb), s));
+  // One pass when every block can own kTileMaxStage staging slots within
+  // kTileOnePassBytes (and the offsets fit int32); otherwise count, scan, fill.
+  const int64_t slots = static_cast<int64_t>(nb) * kTileMaxStage;
+  const bool one_pass =
+      slots * static_cast<int64_t>(sizeof(int32_t)) * (member ? 2 : 1) <= kTileOnePassBytes &&
+      slots <= INT32_MAX;
+  int64_t staged = 0;
+  int32_t h[3];
+  if (one_pass) {
+    LCHECK(dev_reserve(out.stage, sizeof(int32_t) * slots, s));
+    if (member) LCHECK(dev_reserve(out.stage_slot, sizeof(int32_t) * slots, s));
+    k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
+                                        c.list.as<int32_t>(), nullptr, out.desc.as<int32_t>(),
+                                        out.stage.as<int32_t>(),
+                                        member ? out.stage_slot.as<int32_t>() : nullptr,
+                                        out.local.as<uint16_t>(), stats, staged_dev);
+    LTRY(hipGetLastError(), "tile fill kernel");
+    unsigned long long st = 0;
+    LTRY(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LTRY(hipMemcpyAsync(&st, staged_dev, sizeof(st), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (h[0] != 0) return LJGPU_OK;  // does not fit: caller retries smaller or runs untiled
+    staged = static_cast<int64_t>(st);
+    out.stage_stride = kTileMaxStage;
+  } else {
+    LCHECK(dev_reserve(w.tile_ns, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
+    LCHECK(dev_reserve(w.tile_soff, sizeof(int32_t) * (static_cast<size_t>(nb) + 1), s));
+    k_tile_count<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
+                                         c.list.as<int32_t>(), w.tile_ns.as<int32_t>(), stats);
+    LTRY(hipGetLastError(), "tile count kernel");
+    LTRY(hipMemcpyAsync(h, stats, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (h[0] != 0) return LJGPU_OK;  // does not fit: caller retries smaller or runs untiled
+    LCHECK(dev_scan(s, w.tile_ns.as<int32_t>(), w.tile_soff.as<int32_t>(), nb, w, &staged));
+    if (staged > INT32_MAX) return LJGPU_OK;
+    LCHECK(dev_reserve(out.stage, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
+    // The staged particles as slots too, for evaluations over copies of the
+    // positions and momenta in slot order.
+    if (member)
+      LCHECK(dev_reserve(out.stage_slot, sizeof(int32_t) * std::max<int64_t>(staged, 1), s));
+    k_tile_fill<<<grid, kBlock, 0, s>>>(dgeo, nb, mem, slot_of, sptr, c.ptr.as<int32_t>(),
+                                        c.list.as<int32_t>(), w.tile_soff.as<int32_t>(),
+                                        out.desc.as<int32_t>(), out.stage.as<int32_t>(),
+                                        member ? out.stage_slot.as<int32_t>() : nullptr,
+                                        out.local.as<uint16_t>(), stats, staged_dev);
+    LTRY(hipGetLastError(), "tile fill kernel");
+    out.stage_stride = 0;
+  }
   out.tile_particles = T;
   out.blocks = nb;
   out.staged = staged;
