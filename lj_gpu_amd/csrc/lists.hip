@@ -425,11 +425,20 @@ __device__ __forceinline__ int cell_coord(double x, double origin, double width,
   return min(max(v, 0), m - 1);
 }
 
+// Also sets *outside when a periodic box has a coordinate outside [0, box)
+// (the pair search then applies the minimum image in every cell).
 __global__ void k_cell_assign(int32_t n, PosIn q, GridArgs g, int32_t* __restrict__ count,
-                              int32_t* __restrict__ slot, int32_t* __restrict__ cell_of) {
+                              int32_t* __restrict__ slot, int32_t* __restrict__ cell_of,
+                              int32_t* __restrict__ outside) {
   for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
        t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t o = t * q.stride;
+    if (g.box > 0.0) {
+      const double x = q.x[o], y = q.y[o], z = q.z[o];
+      // Written as negations so that NaN counts as outside.
+      if (!(x >= 0.0 && x < g.box && y >= 0.0 && y < g.box && z >= 0.0 && z < g.box))
+        atomicOr(outside, 1);
+    }
     const int cx = cell_coord(q.x[o], g.o0, g.w0, g.nc0, g.box);
     const int cy = cell_coord(q.y[o], g.o1, g.w1, g.nc1, g.box);
     const int cz = cell_coord(q.z[o], g.o2, g.w2, g.nc2, g.box);
@@ -494,11 +503,12 @@ __global__ __launch_bounds__(kBlock) void k_copy_slab(int32_t n, const int32_t* 
 // The reference's pair predicate (cpu_ref/force_aos.cpp:131-135) with the
 // minimum image of the periodic box: every operation rounded separately, in
 // the reference's order, (dx*dx + dy*dy) + dz*dz < SL2.
+template <bool IMAGE = true>
 __device__ __forceinline__ bool within_search(const double4& a, const double4& b, double box,
                                               double half_box, double sl2) {
 #pragma clang fp contract(off)
   double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-  if (box > 0.0) {
+  if (IMAGE && box > 0.0) {
     if (dx > half_box) dx -= box;
     else if (dx < -half_box) dx += box;
     if (dy > half_box) dy -= box;
@@ -593,6 +603,89 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 // 2R+1 cells per side). Same predicate (within_search, in the same argument
 // order), exclusions and MODEs as k_pairs; a row's entries come out in
 // candidate order and are sorted afterwards.
+// The candidates of one cell (see k_pairs_wave), with the minimum image
+// (IMAGE) or without it: a cell whose stencil does not wrap around the box,
+// with every position inside [0, box), has no pair that the image would
+// shift, so the predicate without it gives the same result.
+template <int MODE, bool HALF, int R, bool IMAGE>
+__device__ __forceinline__ void pairs_cell(int32_t p0, int32_t p1, int32_t total, int32_t pre,
+                                           int32_t rs, int32_t excl,
+                                           const double4* __restrict__ cpos, const GridArgs& g,
+                                           double half_box, double sl2, int32_t cap,
+                                           int32_t* __restrict__ nop,
+                                           const int32_t* __restrict__ ptr,
+                                           int32_t* __restrict__ out) {
+  constexpr int kSpan = 2 * R + 1;
+  constexpr int kRuns = 2 * kSpan * kSpan;
+  const int lane = threadIdx.x & 63;
+  for (int32_t g0 = p0; g0 < p1; g0 += 64) {
+    const int32_t mg = min(64, p1 - g0);
+    double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (lane < mg) pi = cpos[g0 + lane];
+    const int32_t iv = static_cast<int32_t>(pi.w);
+    int64_t basev = 0;
+    if (MODE == 1) basev = lane < mg ? ptr[iv] : 0;
+    if (MODE == 2) basev = static_cast<int64_t>(iv) * cap;
+    int32_t cntv = 0;
+    for (int32_t f0 = 0; f0 < total; f0 += 64) {
+      const int32_t f = f0 + lane;
+      const bool live = f < total;
+      // The run holding f: the number of runs ending at or before f.
+      int r = 0;
+#pragma unroll
+      for (int bit = 32; bit > 0; bit >>= 1) {
+        const int t = r + bit;
+        const int32_t v = __shfl(pre, min(t, 64) - 1, 64);
+        if (t <= kRuns && v <= f) r = t;
+      }
+      const int rr = min(r, kRuns - 1);
+      const int32_t u = __shfl(rs, rr, 64) + (f - __shfl(excl, rr, 64));
+      // Lanes past the end read candidate 0 (a valid address) and test nothing.
+      const double4 pj = cpos[live ? u : 0];
+      const int32_t j = static_cast<int32_t>(pj.w);
+      for (int k = 0; k < mg; k++) {
+        double4 a;
+        a.x = readlane_d(pi.x, k);
+        a.y = readlane_d(pi.y, k);
+        a.z = readlane_d(pi.z, k);
+        a.w = 0.0;
+        const int32_t ik = __builtin_amdgcn_readlane(iv, k);
+        const bool hit = live && (HALF ? j > ik : j != ik) &&
+                         within_search<IMAGE>(a, pj, g.box, half_box, sl2);
+        const uint64_t mask = __ballot(hit);
+        if (mask == 0) continue;
+        const int32_t ck = __builtin_amdgcn_readlane(cntv, k);
+        if (MODE != 0 && hit) {
+          const int32_t rank = static_cast<int32_t>(__builtin_amdgcn_mbcnt_hi(
+              static_cast<uint32_t>(mask >> 32),
+              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
+          const int64_t bk =
+              (static_cast<int64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(
+                   static_cast<int32_t>(basev >> 32), k)))
+               << 32) |
+              static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(basev), k));
+          if (MODE == 1 || ck + rank < cap) out[bk + ck + rank] = j;
+        }
+        if (lane == k) cntv = ck + __popcll(mask);
+      }
+    }
+    if (MODE != 1 && lane < mg) nop[iv] = cntv;
+  }
+}
+
+// One wavefront per cell, for cell grids with a stencil (R >= 1). The cell's
+// particles sit one per lane (in groups of 64); the candidates of the
+// (2R+1)^3 cells around it are walked as one flattened sequence, 64 per
+// round, so each candidate is loaded once for all of the cell's particles
+// instead of once per particle. A round's hits for one particle go to
+// consecutive addresses of its row (ballot, then each hit lane's rank among
+// them), so the stores coalesce. The runs of the flattened sequence are the
+// x-rows of the stencil, contiguous in cell order: (2R+1)^2 rows, each split
+// at most once where it wraps around a periodic box (the grid has at least
+// 2R+1 cells per side). Same predicate (within_search, in the same argument
+// order), exclusions and MODEs as k_pairs; a row's entries come out in
+// candidate order and are sorted afterwards. *outside (from k_cell_assign)
+// keeps the minimum image on in every cell.
 template <int MODE, bool HALF, int R>
 __global__ __launch_bounds__(kBlock) void k_pairs_wave(int32_t ncell,
                                                        const double4* __restrict__ cpos,
@@ -600,12 +693,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs_wave(int32_t ncell,
                                                        GridArgs g, double sl2, int32_t cap,
                                                        int32_t* __restrict__ nop,
                                                        const int32_t* __restrict__ ptr,
-                                                       int32_t* __restrict__ out) {
+                                                       int32_t* __restrict__ out,
+                                                       const int32_t* __restrict__ outside) {
   constexpr int kSpan = 2 * R + 1;
   constexpr int kRuns = 2 * kSpan * kSpan;
   static_assert(kRuns <= 64, "one run per lane");
   const int lane = threadIdx.x & 63;
   const bool periodic = g.box > 0.0;
+  const bool all_inside = *outside == 0;
   const double half_box = 0.5 * g.box;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
   for (int64_t c64 = (blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x) >> 6; c64 < ncell;
@@ -652,59 +747,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs_wave(int32_t ncell,
     }
     const int32_t excl = pre - rl;
     const int32_t total = __shfl(pre, 63, 64);
-    for (int32_t g0 = p0; g0 < p1; g0 += 64) {
-      const int32_t mg = min(64, p1 - g0);
-      double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
-      if (lane < mg) pi = cpos[g0 + lane];
-      const int32_t iv = static_cast<int32_t>(pi.w);
-      int64_t basev = 0;
-      if (MODE == 1) basev = lane < mg ? ptr[iv] : 0;
-      if (MODE == 2) basev = static_cast<int64_t>(iv) * cap;
-      int32_t cntv = 0;
-      for (int32_t f0 = 0; f0 < total; f0 += 64) {
-        const int32_t f = f0 + lane;
-        const bool live = f < total;
-        // The run holding f: the number of runs ending at or before f.
-        int r = 0;
-#pragma unroll
-        for (int bit = 32; bit > 0; bit >>= 1) {
-          const int t = r + bit;
-          const int32_t v = __shfl(pre, min(t, 64) - 1, 64);
-          if (t <= kRuns && v <= f) r = t;
-        }
-        const int rr = min(r, kRuns - 1);
-        const int32_t u = __shfl(rs, rr, 64) + (f - __shfl(excl, rr, 64));
-        // Lanes past the end read candidate 0 (a valid address) and test nothing.
-        const double4 pj = cpos[live ? u : 0];
-        const int32_t j = static_cast<int32_t>(pj.w);
-        for (int k = 0; k < mg; k++) {
-          double4 a;
-          a.x = readlane_d(pi.x, k);
-          a.y = readlane_d(pi.y, k);
-          a.z = readlane_d(pi.z, k);
-          a.w = 0.0;
-          const int32_t ik = __builtin_amdgcn_readlane(iv, k);
-          const bool hit = live && (HALF ? j > ik : j != ik) &&
-                           within_search(a, pj, g.box, half_box, sl2);
-          const uint64_t mask = __ballot(hit);
-          if (mask == 0) continue;
-          const int32_t ck = __builtin_amdgcn_readlane(cntv, k);
-          if (MODE != 0 && hit) {
-            const int32_t rank = static_cast<int32_t>(__builtin_amdgcn_mbcnt_hi(
-                static_cast<uint32_t>(mask >> 32),
-                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u)));
-            const int64_t bk =
-                (static_cast<int64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(
-                     static_cast<int32_t>(basev >> 32), k)))
-                 << 32) |
-                static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int32_t>(basev), k));
-            if (MODE == 1 || ck + rank < cap) out[bk + ck + rank] = j;
-          }
-          if (lane == k) cntv = ck + __popcll(mask);
-        }
-      }
-      if (MODE != 1 && lane < mg) nop[iv] = cntv;
-    }
+    const bool wraps = periodic && (!all_inside || cx < R || cx + R >= g.nc0 || cy < R ||
+                                    cy + R >= g.nc1 || cz < R || cz + R >= g.nc2);
+    if (wraps)
+      pairs_cell<MODE, HALF, R, true>(p0, p1, total, pre, rs, excl, cpos, g, half_box, sl2, cap,
+                                      nop, ptr, out);
+    else
+      pairs_cell<MODE, HALF, R, false>(p0, p1, total, pre, rs, excl, cpos, g, half_box, sl2, cap,
+                                       nop, ptr, out);
   }
 }
 
@@ -1341,8 +1391,11 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   int32_t* count = w.cell_count.as<int32_t>();
   int32_t* cstart = w.cell_start.as<int32_t>();
   LCHECK(fill<int32_t>(s, count, ncell, 0));
+  LCHECK(dev_reserve(w.small, sizeof(int64_t) * 8, s));
+  int32_t* outside = w.small.as<int32_t>() + 12;
+  LCHECK(fill<int32_t>(s, outside, 1, 0));
   k_cell_assign<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, q, g, count, w.cell_slot.as<int32_t>(),
-                                                       w.cell_of.as<int32_t>());
+                                                       w.cell_of.as<int32_t>(), outside);
   LTRY(hipGetLastError(), "cell assign kernel");
   int64_t total = 0;
   LCHECK(dev_scan(s, count, cstart, ncell, w, &total));
@@ -1380,7 +1433,7 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   k_pairs<M, H, 0><<<grid, kBlock, 0, s>>>(n, cp, cs, cstart, g, sl2, cap, nop, ptr, outl)
 #define LJ_PAIRS_WAVE(H, RR)                                                                     \
   k_pairs_wave<M, H, RR><<<wave_grid, kBlock, 0, s>>>(static_cast<int32_t>(ncell), cp, cstart, g, \
-                                                      sl2, cap, nop, ptr, outl)
+                                                      sl2, cap, nop, ptr, outl, outside)
     if (half) {
       if (R == 0) LJ_PAIRS(true);
       else if (R == 1) LJ_PAIRS_WAVE(true, 1);

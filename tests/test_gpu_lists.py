@@ -67,6 +67,34 @@ def test_device_build_matches_host_periodic(ctx, k, rho, kind):
         assert info.min_partners == half.number_of_partners.min()
 
 
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
+def test_device_build_wrapped_positions(ctx, k, rho):
+    """Every position inside [0, box): cells whose stencil does not wrap
+    skip the minimum image, which cannot shift any of their pairs."""
+    q, box, _, _ = periodic(k, rho)
+    q = q.copy()
+    q[:, :3] -= box * np.floor(q[:, :3] / box)
+    q[:, :3][q[:, :3] >= box] -= box
+    assert q[:, :3].min() >= 0.0 and q[:, :3].max() < box
+    for kind in (G.LIST_HALF, G.LIST_FULL):
+        host = G.makepair(q, box, kind)
+        got, _ = _device_build(ctx, q, box, kind, G.HALF_ATOMIC)
+        _same(got, host)
+
+
+def test_device_build_positions_outside_the_box(ctx):
+    """Particles moved out by one box length (x in [box, 1.4 box)): the
+    minimum image stays on in every cell, and the list equals the host's."""
+    q, box, _, _ = periodic(12, 1.0)
+    q = q.copy()
+    sel = q[:, 0] < 0.4 * box
+    q[sel, 0] += box
+    for kind in (G.LIST_HALF, G.LIST_FULL):
+        host = G.makepair(q, box, kind)
+        got, _ = _device_build(ctx, q, box, kind, G.HALF_ATOMIC)
+        _same(got, host)
+
+
 @pytest.mark.parametrize("density", [0.5, 1.0])
 def test_device_build_matches_host_reference_box(ctx, density):
     """The reference's own open-box problem (init() with L = 50)."""
