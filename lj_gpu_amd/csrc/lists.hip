@@ -347,7 +347,18 @@ __global__ void k_minmax(int32_t n, const int32_t* __restrict__ a, int32_t* __re
     lo = min(lo, __shfl_xor(lo, d, 64));
     hi = max(hi, __shfl_xor(hi, d, 64));
   }
+  // One atomic pair per workgroup: atomics on one address serialise.
+  __shared__ int32_t slo[kBlock / 64], shi[kBlock / 64];
   if ((threadIdx.x & 63) == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) {
+      lo = min(lo, slo[w]);
+      hi = max(hi, shi[w]);
+    }
     atomicMin(&mm[0], lo);
     atomicMax(&mm[1], hi);
   }
@@ -496,7 +507,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_slab(int32_t n, const int32_t* 
     const int32_t m = len[r];
     const int32_t* in = src + r * stride;
     int32_t* o = dst + start[r];
-    for (int32_t k = lane; k < m; k += 64) o[k] = in[k];
+    // Four loads in flight before their stores.
+    for (int32_t k0 = 0; k0 < m; k0 += 4 * 64) {
+      int32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int32_t k = k0 + u * 64 + lane;
+        v[u] = k < m ? in[k] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int32_t k = k0 + u * 64 + lane;
+        if (k < m) o[k] = v[u];
+      }
+    }
   }
 }
 
@@ -1513,7 +1537,7 @@ int dev_minmax(hipStream_t s, const int32_t* a, int32_t n, ListWork& w, int32_t*
   int32_t* mm = w.small.as<int32_t>();
   LCHECK(fill<int32_t>(s, mm, 1, INT32_MAX));
   LCHECK(fill<int32_t>(s, mm + 1, 1, INT32_MIN));
-  k_minmax<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(n, a, mm);
+  k_minmax<<<grid_for(n, kBlock, 256), kBlock, 0, s>>>(n, a, mm);
   LTRY(hipGetLastError(), "minmax kernel");
   int32_t h[2];
   LTRY(hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");

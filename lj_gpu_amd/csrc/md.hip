@@ -75,9 +75,15 @@ __global__ void k_drift(int32_t n, double* __restrict__ q, const double* __restr
     }
   }
   if (qref) {
+    // One atomic per workgroup: atomics on one address serialise.
+    __shared__ double sm[kBlock / 64];
     for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sm[w]);
       atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
   }
 }
 
