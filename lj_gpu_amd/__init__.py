@@ -604,7 +604,9 @@ class Context:
                precision: int = PREC_F64, accum: int = ACCUM_F64, sample_every: int = 0,
                max_samples: int = 0) -> tuple[MDStats, np.ndarray]:
         """Velocity-Verlet run in place on device AoS q, p; returns the stats
-        and the energy samples (rows: step, U, K, U + K)."""
+        and the energy samples (rows: step, U, K, U + K). In a periodic box
+        no list stays installed afterwards (the run's lists index its
+        internal, spatially sorted copies)."""
         cfg = MDConfig(dt, cutoff, skin, rebuild_every, precision, accum, sample_every)
         st = MDStats()
         cap = max_samples if max_samples > 0 else (steps // sample_every + 2 if sample_every else 2)
