@@ -5,12 +5,16 @@
 // same pair predicate, evaluated unfused so every pair decision is
 // bit-identical to the reference's, and the same CSR (rows ascending in i,
 // partners ascending in j). The O(N^2) pair scan is replaced by a cell list:
-//   1. bin particles into cells at least the search length wide;
-//   2. count each row's partners over the 27 neighbouring cells, scan the
-//      counts into row pointers;
-//   3. write each row's partners, then sort every row.
-// Derived layouts (full list from a half list, ELL, LDS tiles) are built here
-// from the device CSR, so a list never round-trips through the host.
+//   1. bin particles into cells at least half the search length wide (a
+//      radius-2 stencil; the search length and radius 1 for small boxes);
+//   2. search each cell's stencil with one wavefront per cell, writing rows
+//      into a per-row slab (an exact count-then-fill pass when a row
+//      overflows it), and scan the row lengths into row pointers;
+//   3. sort every row into place (or, for the MD loop, copy rows in the
+//      build's own, position-determined order).
+// Derived layouts (full list from a half list, ELL, LDS tiles, and the
+// spatial particle order the tiles may follow) are built here from the
+// device CSR, so a list never round-trips through the host.
 #include "lists.h"
 
 #include <algorithm>
