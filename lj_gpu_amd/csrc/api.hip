@@ -90,7 +90,8 @@ int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
                        ctx->work, &ordered));
   const int32_t* member = ordered ? ctx->work.order.as<int32_t>() : nullptr;
   for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
-    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, member, ctx->work, ctx->tiles));
+    TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, member, ctx->bank_order,
+                       ctx->work, ctx->tiles));
     if (ctx->tiles.tile_particles > 0) break;
   }
   ctx->tiles_for = want;

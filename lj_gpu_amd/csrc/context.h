@@ -67,6 +67,7 @@ struct ljgpu_ctx {
   int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
   int tile_order = LJGPU_TILE_ORDER_SPATIAL;  // requested order of the particles in tiles
   bool tiles_ordered = false;     // the tiling had positions to take that order from
+  bool bank_order = true;         // tile rows in the LDS bank-class order (bank_class_order)
   ljgpu::DevTiles tiles;
   ljgpu::DevMem slot_q, slot_p;   // q and p in slot order during a many-step call
 

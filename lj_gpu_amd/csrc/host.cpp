@@ -173,6 +173,8 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* mem
             std::lower_bound(set.begin(), set.end(), j) - set.begin());
         out->local[sptr[s] + k] = static_cast<uint16_t>(rank * 24);
       }
+      uint16_t buf[kBankRowMax], cnt[kBankClasses], nxt[kBankClasses];
+      bank_class_order(out->local.data() + sptr[s], full.nop[i], s - i0, buf, cnt, nxt);
     }
     i0 += ni;
   }

@@ -141,10 +141,12 @@ int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell);
 // evaluates untiled).
 // `member` (device, one particle per slot, a permutation that keeps every
 // replica's particles within its own slot range) orders the particles the
-// blocks cover; null takes them in index order.
+// blocks cover; null takes them in index order. bank_order puts each row's
+// entries in the LDS bank-class order (bank_class_order), as the host model
+// does; without it rows keep the list order.
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles,
-                   const std::vector<int32_t>& replicas, const int32_t* member, ListWork& w,
-                   DevTiles& out);
+                   const std::vector<int32_t>& replicas, const int32_t* member, bool bank_order,
+                   ListWork& w, DevTiles& out);
 
 // Spatial particle order for LDS tiles, into w.order: each periodic system
 // (the whole set, or every replica of `replicas` with its box) is cut into

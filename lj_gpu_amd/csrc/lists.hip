@@ -1107,6 +1107,26 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
   }
 }
 
+// The bank-class order of every tile row (bank_class_order): one wavefront
+// per tile, one row per lane, each lane's buffers in LDS.
+constexpr int kBankBlock = 64;
+__global__ __launch_bounds__(kBankBlock) void k_bank_order(const int32_t* __restrict__ geo,
+                                                           int32_t nblocks,
+                                                           const int32_t* __restrict__ slot_ptr,
+                                                           uint16_t* __restrict__ local) {
+  __shared__ uint16_t buf[kBankBlock][kBankRowMax];
+  __shared__ uint16_t cnt[kBankBlock][kBankClasses];
+  __shared__ uint16_t nxt[kBankBlock][kBankClasses];
+  const int t = static_cast<int>(threadIdx.x);
+  for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const int32_t i0 = geo[4 * b], ni = geo[4 * b + 1];
+    for (int32_t r = t; r < ni; r += kBankBlock) {
+      const int32_t a = slot_ptr[i0 + r];
+      bank_class_order(local + a, slot_ptr[i0 + r + 1] - a, r, buf[t], cnt[t], nxt[t]);
+    }
+  }
+}
+
 // Batched replicas.
 __global__ void k_copy_add(int64_t n, const int32_t* __restrict__ src, int32_t* __restrict__ dst,
                            int32_t add) {
@@ -1654,7 +1674,7 @@ int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
 }
 
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<int32_t>& reps,
-                   const int32_t* member, ListWork& w, DevTiles& out) {
+                   const int32_t* member, bool bank_order, ListWork& w, DevTiles& out) {
   out.tile_particles = 0;
   out.blocks = 0;
   if (T <= 0 || T > kTileRowsMax || c.n <= 0) return LJGPU_OK;
@@ -1756,6 +1776,10 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
                                         out.local.as<uint16_t>(), stats, staged_dev);
     LTRY(hipGetLastError(), "tile fill kernel");
     out.stage_stride = 0;
+  }
+  if (bank_order) {
+    k_bank_order<<<grid, kBankBlock, 0, s>>>(dgeo, nb, sptr, out.local.as<uint16_t>());
+    LTRY(hipGetLastError(), "tile bank order kernel");
   }
   out.tile_particles = T;
   out.blocks = nb;

@@ -95,6 +95,44 @@ LJ_HD inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
   return ((ns * 24 + 15) / 16) * 16 + ((ne + 14 + 7) / 8) * 16;
 }
 
+// Entry order within a tile row, chosen so that the partner reads of one
+// LDS instruction spread over the banks. The staged position of rank t
+// starts at dword 4 + 6t, so with 64 four-byte banks the bank pair of each of
+// its 8-byte words depends on t mod 32 alone: the entry's class. With four
+// lanes per particle, lane `sub` of the particle at row position r of its
+// tile reads entry k = 16 * batch + 4 * u + sub together with the other 31
+// lanes of its half-wavefront (8 rows). Entry k of the row at position r
+// therefore takes a partner of class (4 * (r & 7) + k) & 31, or, when the row
+// has none of that class left, of the next class up (cyclically) that it
+// has: while the rows' classes last, the 32 lanes read 32 distinct classes.
+// Entries of one class keep their list order. Rows longer than kBankRowMax
+// keep the list order. buf holds kBankRowMax entries, cnt and nxt
+// kBankClasses counts each.
+constexpr int kBankClasses = 32;
+constexpr int kBankRowMax = 256;
+LJ_HD inline void bank_class_order(uint16_t* row, int m, int r, uint16_t* buf, uint16_t* cnt,
+                                   uint16_t* nxt) {
+  if (m < 2 || m > kBankRowMax) return;
+  for (int c = 0; c < kBankClasses; c++) cnt[c] = 0;
+  for (int k = 0; k < m; k++) cnt[(row[k] / 24) & (kBankClasses - 1)]++;
+  int run = 0;
+  for (int c = 0; c < kBankClasses; c++) {
+    nxt[c] = static_cast<uint16_t>(run);
+    run += cnt[c];
+  }
+  for (int k = 0; k < m; k++) {
+    const uint16_t o = row[k];
+    buf[nxt[(o / 24) & (kBankClasses - 1)]++] = o;
+  }
+  // nxt[c] is now the end of class c in buf; its next entry is nxt[c] - cnt[c].
+  for (int k = 0; k < m; k++) {
+    int c = (4 * (r & 7) + k) & (kBankClasses - 1);
+    while (cnt[c] == 0) c = (c + 1) & (kBankClasses - 1);
+    row[k] = buf[nxt[c] - cnt[c]];
+    cnt[c]--;
+  }
+}
+
 // Cell grid of the Verlet-list builders: cells at least the search length
 // wide (so a pair within the search length lies in the same or adjacent
 // cells), at most about 8 per particle. A periodic box with fewer than three

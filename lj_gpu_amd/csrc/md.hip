@@ -286,18 +286,23 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   // copies are already in a spatial order, so consecutive slots are compact
   // tiles without the per-entry slot lookups of spatially ordered tiles over
   // the caller's arrays. The caller's choice is restored on return.
+  // Tiles rebuilt every few steps skip the bank-class row order: the pass
+  // costs more per rebuild than it saves over the kicks between rebuilds.
   struct OrderGuard {
     ljgpu_ctx* c;
     int saved;
+    bool saved_bank;
     ~OrderGuard() {
-      if (c->tile_order != saved) {
+      if (c->tile_order != saved || c->bank_order != saved_bank) {
         c->tile_order = saved;
+        c->bank_order = saved_bank;
         c->tiles_for = -1;
         c->tiles_ordered = false;
       }
     }
-  } order_guard{ctx, ctx->tile_order};
+  } order_guard{ctx, ctx->tile_order, ctx->bank_order};
   ctx->tile_order = LJGPU_TILE_ORDER_INDEX;
+  ctx->bank_order = false;
   const double dt = cfg->dt, cl2 = cfg->cutoff * cfg->cutoff;
   const double search = cfg->cutoff + cfg->skin;
   const bool mixed = cfg->precision == LJGPU_PREC_MIXED;

@@ -112,3 +112,48 @@ def column_order(q: np.ndarray, box: float, tile_particles: int = 64, offsets=No
         keys[a:b] = key0 + (cell[2] * nc[1] + cell[1]) * nc[0] + cell[0]
         key0 += nc[0] * nc[1] * nc[2]
     return np.lexsort((np.arange(n), keys)).astype(np.int32)
+
+
+def bank_class_order(offs: np.ndarray, starts: np.ndarray, lengths: np.ndarray,
+                     pos: np.ndarray) -> np.ndarray:
+    """Numpy restatement of the library's entry order within tile rows.
+
+    offs: LDS byte offsets (24 * staged rank) in list order; row i spans
+    offs[starts[i] : starts[i] + lengths[i]] and sits at position pos[i] of
+    its tile. An entry's class is rank mod 32. Entry k of a row at position r
+    takes the first entry (in list order) of class (4 * (r & 7) + k) mod 32
+    that the row still holds, or of the next class up, cyclically, that it
+    does. Rows shorter than 2 or longer than 256 entries keep the list order.
+    All rows advance together, one entry index per step."""
+    offs = np.asarray(offs)
+    out = offs.copy()
+    starts, lengths, pos = (np.asarray(a, np.int64) for a in (starts, lengths, pos))
+    sel = np.nonzero((lengths >= 2) & (lengths <= 256))[0]
+    if sel.size == 0:
+        return out
+    st, ln, ps = starts[sel], lengths[sel], pos[sel]
+    R = sel.size
+    row_base = np.concatenate([[0], np.cumsum(ln)[:-1]])
+    row_of = np.repeat(np.arange(R), ln)
+    e = np.repeat(st - row_base, ln) + np.arange(int(ln.sum()))
+    vals = offs[e].astype(np.int64)
+    cls = (vals // 24) % 32
+    key = row_of * 32 + cls
+    grouped = vals[np.argsort(key, kind="stable")]  # by row, class, list position
+    cnt = np.bincount(key, minlength=R * 32).reshape(R, 32)
+    cstart = row_base[:, None] + np.concatenate(
+        [np.zeros((R, 1), np.int64), np.cumsum(cnt, 1)[:, :-1]], 1)
+    taken = np.zeros((R, 32), np.int64)
+    rows = np.arange(R)
+    for k in range(int(ln.max())):
+        live = rows[ln > k]
+        c = (4 * (ps[live] % 8) + k) % 32
+        miss = np.nonzero(cnt[live, c] == taken[live, c])[0]
+        if miss.size:  # target class used up: the next class up that is left
+            lm = live[miss]
+            cand = (c[miss][:, None] + np.arange(32)) % 32
+            left = cnt[lm[:, None], cand] - taken[lm[:, None], cand]
+            c[miss] = cand[np.arange(miss.size), (left > 0).argmax(1)]
+        out[st[live] + k] = grouped[cstart[live, c] + taken[live, c]]
+        taken[live, c] += 1
+    return out

@@ -7,7 +7,7 @@ import pytest
 
 import lj_gpu_amd as G
 import oracle as O
-from helpers import column_order, normwise, periodic
+from helpers import bank_class_order, column_order, normwise, periodic
 
 MAX_STAGE = 65535 // 24
 
@@ -47,9 +47,25 @@ def check_tiles(csr, t, tile_particles, member=None):
     partners = csr.sorted_list[start + np.arange(sptr[-1])]
     rank = t.local.astype(np.int64) // 24
     assert np.all(rank < desc[b, 3])
-    np.testing.assert_array_equal(t.stage[desc[b, 2] + rank], partners)
     slot_of = np.empty(n, np.int64)
     slot_of[mem] = np.arange(n)
+    # Each entry's partner rank in its block's staged set, in list order: the
+    # staged sets, keyed (block, slot), are ascending over the whole stage.
+    skey = np.repeat(np.arange(desc.shape[0]), desc[:, 3]) * (n + 1) + slot_of[t.stage]
+    pkey = b * (n + 1) + slot_of[partners]
+    at = np.searchsorted(skey, pkey)
+    np.testing.assert_array_equal(skey[np.minimum(at, skey.size - 1)], pkey)
+    list_offs = (at - desc[b, 2]) * 24
+    # Rows in the bank-class order (checked on about 200 evenly spaced blocks),
+    # each a permutation of its list row (checked on all).
+    row_pos = np.arange(n) - desc[block_of, 0]
+    pick = np.nonzero(block_of % max(1, desc.shape[0] // 200) == 0)[0]
+    expect = bank_class_order(list_offs, sptr[pick], nop_slot[pick], row_pos[pick])
+    sel = np.repeat(sptr[pick] - np.cumsum(np.r_[0, nop_slot[pick]][:-1]), nop_slot[pick]) + \
+        np.arange(int(nop_slot[pick].sum()))
+    np.testing.assert_array_equal(t.local[sel], expect[sel])
+    row_id = np.repeat(np.arange(n, dtype=np.int64), nop_slot) << 16
+    np.testing.assert_array_equal(np.sort(row_id + t.local), np.sort(row_id + list_offs))
     for bi in range(0, desc.shape[0], max(1, desc.shape[0] // 50)):
         s = t.stage[desc[bi, 2]: desc[bi, 2] + desc[bi, 3]]
         assert np.all(np.diff(slot_of[s]) > 0)  # sorted by slot
@@ -209,3 +225,60 @@ def test_make_tiles_ordered_rejects_non_permutations():
     member[5] = full.n
     with pytest.raises(G.LJGPUError):
         G.make_tiles(full, 64, member)
+
+
+def _bank_order_loop(ranks, r):
+    """One row, literally: entry k takes the first remaining partner of class
+    (4 * (r & 7) + k) mod 32, or of the next class up that the row holds."""
+    pools = [[v for v in ranks if v % 32 == c] for c in range(32)]
+    out = []
+    for k in range(len(ranks)):
+        c = (4 * (r % 8) + k) % 32
+        while not pools[c]:
+            c = (c + 1) % 32
+        out.append(pools[c].pop(0))
+    return out
+
+
+def test_bank_class_order_hand_cases():
+    # ranks 0, 32 (class 0) and 1, 33 (class 1); offsets are 24 * rank
+    row = np.array([0, 32, 1, 33]) * 24
+    one = lambda r: list(bank_class_order(row, [0], [4], [r]) // 24)
+    assert one(0) == [0, 1, 32, 33]   # targets 0, 1, then 2 and 3 wrap to 0 and 1
+    assert one(1) == [0, 32, 1, 33]   # targets 4..7 all wrap to class 0 first
+    assert one(8) == one(0)           # only r & 7 matters
+    # rows of one entry, and rows longer than 256, keep the list order
+    long = np.arange(300)[::-1] * 24
+    np.testing.assert_array_equal(bank_class_order(long, [0], [300], [3]), long)
+    np.testing.assert_array_equal(bank_class_order(np.array([48]), [0], [1], [5]), [48])
+
+
+def test_bank_class_order_matches_the_literal_rule():
+    rng = np.random.default_rng(11)
+    lengths = rng.integers(0, 258, 60)
+    starts = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    pos = rng.integers(0, 64, lengths.size)
+    offs = rng.integers(0, 2700, int(lengths.sum())) * 24
+    got = bank_class_order(offs, starts, lengths, pos)
+    for a, m, r in zip(starts, lengths, pos):
+        row = offs[a:a + m] // 24
+        want = _bank_order_loop(list(row), r) if 2 <= m <= 256 else list(row)
+        assert list(got[a:a + m] // 24) == want
+
+
+def test_bank_class_order_gives_distinct_classes_per_half_wave():
+    # Eight rows that each hold every class at least 4 times: for the first
+    # 128 entries the 32 lanes of a half-wavefront (rows r = 0..7, entries
+    # 4u..4u+3) read 32 distinct classes.
+    rng = np.random.default_rng(5)
+    rows = [rng.permutation(np.concatenate([np.arange(32) + 32 * rng.integers(0, 80, 32)
+                                            for _ in range(5)])) for _ in range(8)]
+    offs = np.concatenate(rows) * 24
+    lengths = np.full(8, 160)
+    got = bank_class_order(offs, np.arange(8) * 160, lengths, np.arange(8)).reshape(8, 160) // 24
+    for base in range(0, 128, 4):
+        cls = (got[:, base:base + 4] % 32).ravel()
+        assert np.unique(cls).size == 32
+    # and every row is a permutation of its list row
+    for r in range(8):
+        np.testing.assert_array_equal(np.sort(got[r]), np.sort(rows[r]))
