@@ -210,8 +210,10 @@ int ljgpu_set_tiling(ljgpu_ctx* ctx, int32_t tile_particles, int stage_offsets,
  * ljgpu_make_tiles produces (derived on the device for the current kernel
  * configuration; *n_blocks = 0 when the kernels run untiled). Pass null
  * outputs to query the sizes first; local needs one value per entry of the
- * evaluated list. Inspection only; the tiling's counterpart in the reference
- * is the derived transposed list (cuda/force_cuda.cu:229-240). */
+ * evaluated list. The list of a periodic ljgpu_md_run is tiled without the
+ * bank-class row order, but that run leaves no list installed. Inspection
+ * only; the tiling's counterpart in the reference is the derived transposed
+ * list (cuda/force_cuda.cu:229-240). */
 int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_t* stage,
                     int64_t stage_capacity, uint16_t* local, int64_t local_capacity,
                     int32_t* n_blocks, int64_t* n_stage);
@@ -420,9 +422,14 @@ int ljgpu_make_transposed(int32_t particle_number, const int32_t* number_of_part
  * per block (first particle, particle count, staging offset, staging count,
  * first list entry, entry count, replica (0 here), middle particle = first
  * particle + count / 2), stage the staged particle indices,
- * local (n_entries values, parallel to sorted_list) the LDS byte offset
- * 24*rank of each entry's j within its block's staging area; max_stage the
- * largest staging count. Pass null outputs to query the sizes first. */
+ * local (n_entries values, row by row as in sorted_list) the LDS byte
+ * offset 24*rank of each entry's j within its block's staging area; max_stage
+ * the largest staging count. Within a row of at most 256 entries the offsets
+ * are permuted so that the partner reads of one LDS instruction spread over
+ * the banks: entry k of the row at position r of its block takes a partner
+ * of rank class (4*(r & 7) + k) mod 32, or of the next class up that the row
+ * still holds, entries of one class keeping their list order (longer rows
+ * keep the list order). Pass null outputs to query the sizes first. */
 int ljgpu_make_tiles(int32_t particle_number, const int32_t* number_of_partners,
                      const int32_t* pointer, const int32_t* sorted_list, int64_t n_entries,
                      int32_t tile_particles, int32_t* desc, int64_t desc_capacity,
