@@ -306,8 +306,18 @@ def test_md_slot_order_copies_return_to_caller_order(ctx):
     with pytest.raises(G.LJGPUError) as e:
         ctx.list_info()
     assert e.value.code == G.ERR_NO_LIST
-    # the caller's tile order is restored
-    ctx.set_list(G.makepair(q0, box, G.LIST_FULL))
+    # the caller's tile order is restored, and the bank-class row order with
+    # it (the run tiles its own lists without it): the device tiles of a new
+    # list equal the host model's
+    full = G.makepair(q0, box, G.LIST_FULL)
+    ctx.set_list(full)
     dq, dp = ctx.array(q0), ctx.zeros(q0.shape)
     ctx.force_aos_f64(dq, dp, DT, G.CL2, box, 1)
-    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    order, member = ctx.get_tile_order()
+    assert order == G.TILE_ORDER_SPATIAL
+    t = ctx.get_tiles()
+    info = ctx.list_info()
+    h = G.make_tiles(full, info.tile_particles, member)
+    assert np.all(h.desc[:-1, 1] == info.tile_particles)
+    np.testing.assert_array_equal(t.desc, h.desc)
+    np.testing.assert_array_equal(t.local, h.local)
