@@ -221,9 +221,11 @@ int ljgpu_get_tiles(ljgpu_ctx* ctx, int32_t* desc, int64_t desc_capacity, int32_
 /* Order of the particles in tiles. LJGPU_TILE_ORDER_SPATIAL (the default)
  * sorts each periodic system into columns along x, about 0.75 *
  * cbrt(tile_particles / density) wide, and each column into x bins of about
- * one particle, ordered by (column, bin, index). A block of consecutive
- * particles in that order is a compact group that stages fewer partner
- * positions than a block of consecutive indices. The order needs positions:
+ * one particle, ordered by (column, bin, index); then, within each block of
+ * tile_particles consecutive slots, by row length (longest first, ties kept
+ * in order), so that a wavefront's rows need similar numbers of batches. A
+ * block of consecutive particles in that order is a compact group that
+ * stages fewer partner positions than a block of consecutive indices. The order needs positions:
  * ljgpu_build_list takes it from its own, a list installed with
  * ljgpu_set_list_* from the first force evaluation (index order until
  * then). Open boxes, and bins of more than 256 particles, keep index order.

@@ -88,6 +88,7 @@ int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
   if (spatial && q && ctx->eval.n > 0)
     TRY(dev_tile_order(ctx->stream, *q, ctx->eval.n, box, ctx->rep_off, ctx->rep_boxes, want,
                        ctx->work, &ordered));
+  if (ordered) TRY(dev_length_sort(ctx->stream, ctx->eval, want, ctx->rep_off, ctx->work));
   const int32_t* member = ordered ? ctx->work.order.as<int32_t>() : nullptr;
   for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
     TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, member, ctx->bank_order,

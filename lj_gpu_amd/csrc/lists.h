@@ -160,6 +160,13 @@ int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
                    const std::vector<int32_t>& replicas, const std::vector<double>& boxes,
                    int32_t tile_particles, ListWork& w, bool* ok);
 
+// Within each block of tile_particles consecutive slots of w.order (restarted
+// at every replica boundary, as dev_make_tiles forms its blocks), the slots
+// reordered by the row length of their particles in c, longest first, ties
+// in the previous order.
+int dev_length_sort(hipStream_t s, const DevCSR& c, int32_t tile_particles,
+                    const std::vector<int32_t>& replicas, ListWork& w);
+
 // Batched replicas: lists of every replica (particles [off[r], off[r+1]),
 // box boxes[r]) built on the device into one CSR with global indices.
 int dev_makepair_batch(hipStream_t s, const PosIn& q, const std::vector<int32_t>& off,

@@ -1223,6 +1223,34 @@ __global__ void k_inverse(int32_t n, const int32_t* __restrict__ member,
     slot_of[member[t]] = static_cast<int32_t>(t);
 }
 
+// Each group of slots (geo: first slot, count) reordered by the row length
+// of its particles, longest first, ties in the previous slot order: the
+// wavefronts of a tile then hold rows of similar length, which need similar
+// numbers of batches.
+__global__ __launch_bounds__(kBlock) void k_group_length_sort(const int32_t* __restrict__ geo,
+                                                              int32_t ngroups,
+                                                              const int32_t* __restrict__ nop,
+                                                              int32_t* __restrict__ order) {
+  __shared__ int32_t mem[kTileRowsMax];
+  __shared__ int32_t len[kTileRowsMax];
+  for (int32_t b = blockIdx.x; b < ngroups; b += gridDim.x) {
+    const int32_t i0 = geo[4 * b], ni = geo[4 * b + 1];
+    for (int32_t t = threadIdx.x; t < ni; t += blockDim.x) {
+      const int32_t i = order[i0 + t];
+      mem[t] = i;
+      len[t] = nop[i];
+    }
+    __syncthreads();
+    for (int32_t t = threadIdx.x; t < ni; t += blockDim.x) {
+      const int32_t l = len[t];
+      int32_t r = 0;
+      for (int32_t u = 0; u < ni; u++) r += len[u] > l || (len[u] == l && u < t);
+      order[i0 + r] = mem[t];
+    }
+    __syncthreads();
+  }
+}
+
 // Row length of the particle in each slot.
 __global__ void k_slot_nop(int32_t n, const int32_t* __restrict__ member,
                            const int32_t* __restrict__ nop, int32_t* __restrict__ out) {
@@ -1670,6 +1698,29 @@ int dev_tile_order(hipStream_t s, const PosIn& q, int32_t n, double box,
                                                          w.order.as<int32_t>());
   LTRY(hipGetLastError(), "tile order kernels");
   *ok = true;
+  return LJGPU_OK;
+}
+
+int dev_length_sort(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<int32_t>& reps,
+                    ListWork& w) {
+  if (T <= 0 || T > kTileRowsMax || c.n <= 0) return LJGPU_OK;
+  // The groups are dev_make_tiles's blocks of T consecutive slots.
+  std::vector<int32_t> geo;
+  const std::vector<int32_t> single = {0, c.n};
+  const std::vector<int32_t>& off = reps.empty() ? single : reps;
+  for (size_t r = 0; r + 1 < off.size(); r++)
+    for (int32_t i0 = off[r]; i0 < off[r + 1]; i0 += T)
+      geo.insert(geo.end(), {i0, std::min(T, off[r + 1] - i0), 0, 0});
+  const int32_t ng = static_cast<int32_t>(geo.size() / 4);
+  if (ng == 0) return LJGPU_OK;
+  LCHECK(dev_reserve(w.tile_geo, sizeof(int32_t) * geo.size(), s));
+  LTRY(hipMemcpyAsync(w.tile_geo.p, geo.data(), sizeof(int32_t) * geo.size(),
+                      hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  k_group_length_sort<<<std::min(ng, 65535), kBlock, 0, s>>>(
+      w.tile_geo.as<int32_t>(), ng, c.nop.as<int32_t>(), w.order.as<int32_t>());
+  LTRY(hipGetLastError(), "tile length sort kernel");
+  // The upload's host vector must outlive the copy.
+  LTRY(hipStreamSynchronize(s), "hipStreamSynchronize");
   return LJGPU_OK;
 }
 

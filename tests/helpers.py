@@ -81,11 +81,13 @@ def to_soa(q4: np.ndarray) -> np.ndarray:
 
 def column_order(q: np.ndarray, box: float, tile_particles: int = 64, offsets=None,
                  boxes=None) -> np.ndarray:
-    """Numpy restatement of the library's spatial tile order: each periodic
-    system cut into columns along x about 0.75 * cbrt(T / density) wide in y
-    and z (the count per side rounded to the nearest integer), each column into x bins of about one particle (the bin of a
-    coordinate is floor(wrapped / width), clamped), particles ordered by
-    (system, z column, y column, x bin, index)."""
+    """Numpy restatement of the first step of the library's spatial tile
+    order: each periodic system cut into columns along x about
+    0.75 * cbrt(T / density) wide in y and z (the count per side rounded to
+    the nearest integer), each column into x bins of about one particle (the
+    bin of a coordinate is floor(wrapped / width), clamped), particles ordered
+    by (system, z column, y column, x bin, index). length_sorted applies the
+    second step."""
     n = q.shape[0]
     off = [0, n] if offsets is None else [int(v) for v in offsets]
     bx = [box] if offsets is None else [float(v) for v in boxes]
@@ -157,3 +159,19 @@ def bank_class_order(offs: np.ndarray, starts: np.ndarray, lengths: np.ndarray,
         out[st[live] + k] = grouped[cstart[live, c] + taken[live, c]]
         taken[live, c] += 1
     return out
+
+
+def length_sorted(member: np.ndarray, nop: np.ndarray, tile_particles: int,
+                  offsets=None) -> np.ndarray:
+    """The library's spatial order after its second step: within each block of
+    tile_particles consecutive slots (restarted at every system boundary),
+    slots reordered by the row length nop of their particles, longest first,
+    ties kept in order."""
+    member = np.array(member, dtype=np.int32)
+    nop = np.asarray(nop)
+    off = [0, member.size] if offsets is None else [int(v) for v in offsets]
+    for a, b in zip(off[:-1], off[1:]):
+        for i0 in range(a, b, tile_particles):
+            seg = member[i0:min(i0 + tile_particles, b)].copy()
+            member[i0:i0 + seg.size] = seg[np.argsort(-nop[seg], kind="stable")]
+    return member

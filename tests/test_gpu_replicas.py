@@ -7,7 +7,7 @@ import pytest
 
 import lj_gpu_amd as G
 import oracle as O
-from helpers import FP64_TOL, PBC_DT, column_order, normwise
+from helpers import FP64_TOL, PBC_DT, column_order, length_sorted, normwise
 
 pytestmark = pytest.mark.gpu
 
@@ -67,7 +67,9 @@ def test_batched_spatial_tile_order(ctx, work):
     ctx.build_list(ctx.array(q), q.shape[0], 0.0, G.LIST_FULL)
     order, member = ctx.get_tile_order()
     assert order == G.TILE_ORDER_SPATIAL
-    np.testing.assert_array_equal(member, column_order(q, 0.0, 32, off, boxes))
+    nop = ctx.get_list().number_of_partners
+    np.testing.assert_array_equal(
+        member, length_sorted(column_order(q, 0.0, 32, off, boxes), nop, 32, off))
     t = ctx.get_tiles()
     for i0, ni, *_rest in t.desc:
         r = np.searchsorted(off, i0, side="right") - 1

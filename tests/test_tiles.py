@@ -7,7 +7,7 @@ import pytest
 
 import lj_gpu_amd as G
 import oracle as O
-from helpers import bank_class_order, column_order, normwise, periodic
+from helpers import bank_class_order, column_order, length_sorted, normwise, periodic
 
 MAX_STAGE = 65535 // 24
 
@@ -282,3 +282,12 @@ def test_bank_class_order_gives_distinct_classes_per_half_wave():
     # and every row is a permutation of its list row
     for r in range(8):
         np.testing.assert_array_equal(np.sort(got[r]), np.sort(rows[r]))
+
+
+def test_length_sorted_hand_cases():
+    # blocks of 4 slots: lengths 9, 5, 2, 7 -> longest first; the last block
+    # is partial
+    nop = np.array([5, 7, 7, 2, 9, 1])
+    assert list(length_sorted([4, 0, 3, 1, 2, 5], nop, 4)) == [4, 1, 0, 3, 2, 5]
+    # ties keep their order, and blocks restart at each system boundary
+    assert list(length_sorted([0, 1, 2, 3], np.array([3, 3, 5, 3]), 4, [0, 1, 4])) == [0, 2, 1, 3]
