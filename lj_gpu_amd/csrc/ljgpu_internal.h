@@ -106,8 +106,9 @@ LJ_HD inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
 // has none of that class left, of the next class up (cyclically) that it
 // has: while the rows' classes last, the 32 lanes read 32 distinct classes.
 // Entries of one class keep their list order. Rows longer than kBankRowMax
-// keep the list order. buf holds kBankRowMax entries, cnt and nxt
-// kBankClasses counts each.
+// keep the list order. Kernels with other lane counts read the same rows
+// with a weaker spread; the result is the same set of pairs either way.
+// buf holds kBankRowMax entries, cnt and nxt kBankClasses counts each.
 constexpr int kBankClasses = 32;
 constexpr int kBankRowMax = 256;
 LJ_HD inline void bank_class_order(uint16_t* row, int m, int r, uint16_t* buf, uint16_t* cnt,
