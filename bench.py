@@ -10,16 +10,20 @@ evaluation p += F(q) * dt over the whole list (the body of the reference's
 measure() loop, cuda/force_cuda.cu:333-335).
 
 Prints ONE JSON line on stdout:
-  value          unique pairs * steps / GPU time of the FP64 kernel (Gpair/s),
-                 HIP events on the library's stream around exactly K steps
+  value          unique pairs * steps / GPU time of the FP64 kernel (Gpair/s):
+                 the median of R timed calls of exactly K steps, each between
+                 HIP events on the library's stream, after W warm-up steps, one
+                 untimed call of K steps and a short untimed settling period
   mixed          the same for the mixed-precision kernel, with its force error
                  against the FP64 kernel (max and RMS, normwise)
-  roofline       algorithmic bytes per launch / average launch time, vs 8 TB/s
-  cpu_baseline   the oracle's force_sorted (the reference's CPU path restated:
-                 half list, j scatter) on one host core, a bounded sample of the
-                 same list
+  roofline       algorithmic bytes per launch / median launch time, vs 8 TB/s
+  cpu_baseline   the reference's force_sorted (half list, j scatter) restated
+                 in oracle/cpu_baseline.cpp (-O3, AVX2/FMA), on one host core,
+                 a bounded sample of the same list; cpu_baselines_other holds
+                 the restated force_intrin_mat_transpose (one core) and an
+                 all-core OpenMP full-list loop
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--repeats R]
        python bench.py --workload {large,replicas,md} [--steps K]
 The other workloads print one JSON line each in the same format: "large" the
 same evaluation on FCC 4*64^3 = 1,048,576 particles with FP64, mixed and
@@ -79,21 +83,81 @@ def measured_traffic(info, tile_order):
     return t.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(q, box, half, dt, budget_s: float) -> dict:
-    """The oracle's force_sorted on one core over the same periodic list (a
-    bounded sample of steps, about `budget_s` seconds)."""
+def cpu_baselines(q, box, half, full, dt, budget_s: float) -> list[dict]:
+    """The reference's CPU force path restated (oracle/cpu_baseline.cpp, -O3
+    AVX2/FMA), timed on this host over the same periodic lists, each on a
+    bounded sample of steps (about `budget_s` seconds): force_sorted on one
+    core (the reference's "Reference" row), force_intrin_mat_transpose on
+    one core (its fastest CPU routine) and the full list over all cores with
+    OpenMP (labelled; the reference has no threaded CPU path)."""
+    import numpy as np
+
     import oracle as O
 
-    csr = O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half")
-    t1 = O.time_force_sorted_aos(csr, q, dt, O.CL2, box, steps=1)
-    steps = max(1, min(1000, int(budget_s / max(t1, 1e-6))))
-    t = O.time_force_sorted_aos(csr, q, dt, O.CL2, box, steps=steps)
-    rate = half.n_entries * steps / t / 1e9
-    return {"value": round(rate, 6), "unit": "Gpair-interactions/s", "cores": 1, "kind": "port",
-            "sample": f"oracle force_sorted (reference cpu_ref/force_aos.cpp:192-224 restated, "
-                      f"-O2, minimum image) over the same N={half.n} half list "
-                      f"({half.n_entries} pairs), {steps} steps in {t:.2f} s, 1 thread",
-            "cpu": cpu_model(), "seconds": round(t, 3), "steps": steps}
+    h = O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half")
+    f = O.CSR(full.number_of_partners, full.pointer, full.sorted_list, "full")
+    qa = O.aligned_zeros(q.shape)
+    qa[...] = q
+    L = O.baseline_lib()
+    build = L.cb_build_info().decode()
+    threads = int(L.cb_threads())
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    out = []
+    for kind, csr, name, cores, ref in (
+            (O.BASELINE_SORTED, h, "force_sorted", 1, "cpu_ref/force_aos.cpp:192-224"),
+            (O.BASELINE_MAT_TRANSPOSE, h, "force_intrin_mat_transpose", 1,
+             "cpu_ref/force_aos.cpp:413-500"),
+            (O.BASELINE_FULL_OMP, f, "full list, OpenMP over rows", threads,
+             "cpu_ref/force_aos.cpp:192-224 on the full list")):
+        p = O.aligned_zeros(q.shape)
+        t1 = O.run_baseline(kind, csr, qa, p, dt, O.CL2, box, 1)
+        steps = max(1, min(1000, int(budget_s / max(t1, 1e-6))))
+        t = O.run_baseline(kind, csr, qa, p, dt, O.CL2, box, steps)
+        rate = half.n_entries * steps / t / 1e9
+        out.append({
+            "value": round(rate, 6), "unit": "Gpair-interactions/s", "cores": cores,
+            "kind": "port", "routine": name,
+            "sample": f"{name} ({ref} restated, minimum image) over the same N={half.n} "
+                      f"{'half' if csr is h else 'full'} list ({half.n_entries} unique pairs), "
+                      f"{steps} steps in {t:.2f} s, {cores} thread(s)",
+            "cpu": cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "build": build, "seconds": round(t, 3), "steps": steps})
+        log(f"cpu baseline {name}: {rate:.4f} Gpair/s ({cores} thread(s))")
+    return out
+
+
+def time_leg(ctx, fn, steps: int, warmup: int, repeats: int, settle_ms: float) -> dict:
+    """Time `fn(steps)` (one force call of `steps` launches) the way the
+    reference's measure() times its launch loop (cuda/force_cuda.cu:330-336):
+    only the launches, after every one-time cost. `warmup` untimed steps,
+    then one untimed call of exactly `steps` steps (so the graphs and buffers
+    the timed calls use already exist), then untimed calls of `steps` until
+    `settle_ms` of GPU time has passed (clocks settle), then `repeats` timed
+    calls, each bracketed by HIP events on the library's stream. Returns the
+    per-step times of the repeats (ms) and their median, min and max."""
+    def timed():
+        ctx.sync()
+        ctx.timer_start()
+        fn(steps)
+        ms = ctx.timer_stop()
+        ctx.sync()
+        return ms
+
+    if warmup > 0:
+        fn(warmup)
+    fn(steps)
+    ctx.sync()
+    settled = 0.0
+    while settled < settle_ms:
+        settled += timed()
+    per_step = sorted(timed() / steps for _ in range(repeats))
+    med = per_step[len(per_step) // 2] if len(per_step) % 2 else \
+        0.5 * (per_step[len(per_step) // 2 - 1] + per_step[len(per_step) // 2])
+    return {"ms_per_step": med, "min_ms_per_step": per_step[0], "max_ms_per_step": per_step[-1],
+            "repeats_ms_per_step": [round(v, 7) for v in per_step], "settle_ms": round(settled, 1)}
 
 
 def bench_large(args, G, np) -> dict:
@@ -113,12 +177,9 @@ def bench_large(args, G, np) -> dict:
                                                                     G.ACCUM_F64)),
                      ("mixed_kahan", lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s,
                                                                    G.ACCUM_KAHAN_F32))):
-        fn(args.warmup)
-        ctx.sync()
-        ctx.timer_start()
-        fn(args.steps)
-        ms = ctx.timer_stop()
-        out[name] = {"ms_per_step": ms / args.steps, "gpair_s": e / 2 * args.steps / ms / 1e6}
+        t = time_leg(ctx, fn, args.steps, args.warmup, args.repeats, args.settle_ms)
+        out[name] = {"ms_per_step": t["ms_per_step"], "gpair_s": e / 2 / t["ms_per_step"] / 1e6,
+                     "timing": t}
     errs = {}
     dF = ctx.zeros(q.shape)
     ctx.force_aos_f64(dq, dF, 1.0, G.CL2, box, 1)
@@ -172,11 +233,8 @@ def bench_replicas(args, G, np) -> dict:
     for name, fn in (("f64", lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, 0.0, s)),
                      ("mixed", lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, 0.0, s,
                                                              G.ACCUM_F64))):
-        fn(args.warmup)
-        ctx.sync()
-        ctx.timer_start()
-        fn(args.steps)
-        res[name] = ctx.timer_stop() / args.steps
+        res[name] = time_leg(ctx, fn, args.steps, args.warmup, args.repeats,
+                             args.settle_ms)["ms_per_step"]
     ctx.close()
     ms = res["f64"]
     return {
@@ -249,10 +307,13 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 5000; 1000 for the md workload)")
-    # At least one graph chunk (100 launches), so that no capture falls in
-    # the timed region, and long enough for the clocks to settle.
-    ap.add_argument("--warmup", type=int, default=1000)
+                    help="timed steps per repeat (default 1000; 1000 for the md workload)")
+    ap.add_argument("--warmup", type=int, default=100, help="untimed warm-up steps")
+    ap.add_argument("--repeats", type=int, default=7,
+                    help="timed calls of --steps steps; the median is reported (at least 5)")
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="untimed calls of --steps steps before timing, until this much GPU "
+                         "time has passed (clock settling)")
     ap.add_argument("--cells", type=int, default=31)
     ap.add_argument("--density", type=float, default=1.0)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per particle (0: library default)")
@@ -261,7 +322,8 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=-1, help="particles per LDS tile (-1: default)")
     ap.add_argument("--tile-order", choices=["spatial", "index"], default="spatial",
                     help="order of the particles in LDS tiles")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="seconds of CPU work per CPU baseline (three baselines)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["all", "f64", "mixed"], default="all")
     ap.add_argument("--workload", choices=["force", "large", "replicas", "md"], default="force",
@@ -278,7 +340,8 @@ def main() -> int:
                          "(a known-size kernel that calibrates rocprofv3's byte counters)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 1000 if args.workload == "md" else 5000
+        args.steps = 1000
+    args.repeats = max(args.repeats, 5)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -316,28 +379,19 @@ def main() -> int:
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)
 
-    def timed(fn, steps):
-        ctx.sync()
-        ctx.timer_start()
-        fn(steps)
-        ms = ctx.timer_stop()
-        ctx.sync()
-        return ms
-
     f64 = lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, box, s)
     mixed = lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s, G.ACCUM_F64)
 
-    result = {}
-    ms_f64 = None
+    t_f64 = t_mixed = None
     if args.only in ("all", "f64"):
-        f64(args.warmup)
-        ms_f64 = timed(f64, args.steps)
-        log(f"fp64: {args.steps} steps in {ms_f64:.3f} ms")
-    ms_mixed = None
+        t_f64 = time_leg(ctx, f64, args.steps, args.warmup, args.repeats, args.settle_ms)
+        log(f"fp64: {args.steps} steps, median {t_f64['ms_per_step'] * 1e3:.3f} us/step "
+            f"(min {t_f64['min_ms_per_step'] * 1e3:.3f}, max {t_f64['max_ms_per_step'] * 1e3:.3f})")
     if args.only in ("all", "mixed"):
-        mixed(args.warmup)
-        ms_mixed = timed(mixed, args.steps)
-        log(f"mixed: {args.steps} steps in {ms_mixed:.3f} ms")
+        t_mixed = time_leg(ctx, mixed, args.steps, args.warmup, args.repeats, args.settle_ms)
+        log(f"mixed: {args.steps} steps, median {t_mixed['ms_per_step'] * 1e3:.3f} us/step "
+            f"(min {t_mixed['min_ms_per_step'] * 1e3:.3f}, "
+            f"max {t_mixed['max_ms_per_step'] * 1e3:.3f})")
     # The tiling in use (spatially ordered at the first evaluation).
     info = ctx.list_info()
     tile_order = ("spatial" if ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL else "index")
@@ -354,9 +408,9 @@ def main() -> int:
     err_rms = float(np.sqrt(((M - F) ** 2).mean()) / scale)
 
     bytes_per_launch = 4 * e + 104 * n  # list + (nop, pointer) + q + p read/write, AoS double4
-    ms_main = ms_f64 if ms_f64 is not None else ms_mixed
-    per_launch_s = ms_main / 1e3 / args.steps
-    value = pairs * args.steps / (ms_main / 1e3) / 1e9
+    main_t = t_f64 if t_f64 is not None else t_mixed
+    per_launch_s = main_t["ms_per_step"] / 1e3
+    value = pairs / per_launch_s / 1e9
     achieved = bytes_per_launch / per_launch_s / 1e9
     result = {
         "metric": METRIC,
@@ -365,11 +419,11 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_main / args.steps, 6),
+        "ms_per_step": round(main_t["ms_per_step"], 7),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if ms_f64 is not None else "f32/f64-mixed",
+        "dtype": "f64" if t_f64 is not None else "f32/f64-mixed",
         "data": "synthetic",
         "config": {
             "workload": f"LJ force, FCC 4*{args.cells}^3={n} particles, density {args.density}, "
@@ -383,6 +437,16 @@ def main() -> int:
             "tile_max_stage": info.tile_max_stage,
             "tile_order": tile_order,
         },
+        "timing": {
+            "method": f"HIP events on the library stream around each of {args.repeats} timed "
+                      f"calls of {args.steps} steps; median reported. Untimed before them: "
+                      f"{args.warmup} warm-up steps, one call of {args.steps} steps, then "
+                      f"calls of {args.steps} steps for {main_t['settle_ms']} ms",
+            "median_ms_per_step": round(main_t["ms_per_step"], 7),
+            "min_ms_per_step": round(main_t["min_ms_per_step"], 7),
+            "max_ms_per_step": round(main_t["max_ms_per_step"], 7),
+            "repeats_ms_per_step": main_t["repeats_ms_per_step"],
+        },
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -394,10 +458,13 @@ def main() -> int:
             "avg_launch_us": round(per_launch_s * 1e6, 3),
             "flop_rate_tflops": round(FLOP_PER_ENTRY * e / per_launch_s / 1e12, 3),
         },
-        "mixed": None if ms_mixed is None else {
-            "value": round(pairs * args.steps / (ms_mixed / 1e3) / 1e9, 3),
-            "ms_per_step": round(ms_mixed / args.steps, 6),
-            "speedup_vs_f64": None if ms_f64 is None else round(ms_f64 / ms_mixed, 4),
+        "mixed": None if t_mixed is None else {
+            "value": round(pairs / (t_mixed["ms_per_step"] / 1e3) / 1e9, 3),
+            "ms_per_step": round(t_mixed["ms_per_step"], 7),
+            "min_ms_per_step": round(t_mixed["min_ms_per_step"], 7),
+            "max_ms_per_step": round(t_mixed["max_ms_per_step"], 7),
+            "speedup_vs_f64": None if t_f64 is None else
+            round(t_f64["ms_per_step"] / t_mixed["ms_per_step"], 4),
             "accum": "f64",
             "max_rel_force_err_vs_f64": err_max,
             "rms_rel_force_err_vs_f64": err_rms,
@@ -413,8 +480,9 @@ def main() -> int:
         a.free()
         b.free()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(q, box, half, dt, args.cpu_seconds)
-        log(f"cpu baseline: {result['cpu_baseline']['value']} Gpair/s")
+        base = cpu_baselines(q, box, half, full, dt, args.cpu_seconds)
+        result["cpu_baseline"] = base[0]
+        result["cpu_baselines_other"] = base[1:]
     else:
         result["cpu_baseline"] = None
     ctx.close()

@@ -370,8 +370,16 @@ int ljgpu_maxwell_boltzmann(int32_t particle_number, double temperature, uint32_
 
 /* Run repeated force evaluations (steps >= 4) and the MD driver's plain steps
  * as replays of captured HIP graphs (1, the default) or as individual
- * launches (0). */
+ * launches (0). A force call of `steps` launches replays graphs of 64, 16 and
+ * 4 launches (longest first) and launches the remaining 0-3 singly; each
+ * graph is captured once per argument set and kept in a cache of 12 (least
+ * recently used evicted). */
 int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable);
+
+/* Graph cache state: graphs captured over the context's life and graphs
+ * cached now. No counterpart in the reference (which has no graphs); lets a
+ * caller check that repeated calls replay instead of re-capturing. */
+int ljgpu_graph_stats(ljgpu_ctx* ctx, int64_t* captures, int32_t* cached);
 
 /* ---- host-side setup (CPU only, no device needed) ----------------------- */
 /* The reference's init(): FCC lattice in an open box of side box_length,

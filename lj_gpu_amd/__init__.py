@@ -177,6 +177,7 @@ def lib() -> C.CDLL:
         "ljgpu_energy": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
         "ljgpu_maxwell_boltzmann": [_i32, _d, C.c_uint32, _vp],
         "ljgpu_set_graphs": [_vp, C.c_int],
+        "ljgpu_graph_stats": [_vp, C.POINTER(_i64), C.POINTER(_i32)],
         "ljgpu_pair_dat_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64, _i32, _i64],
         "ljgpu_pair_dat_read": [C.c_char_p, _i32, _i32, _i64, _vp, _vp, _vp, _i64,
                                 C.POINTER(_i64)],
@@ -206,6 +207,15 @@ def _check(rc: int) -> None:
     if rc != OK:
         msg = lib().ljgpu_last_error()
         raise LJGPUError(rc, msg.decode() if msg else "")
+
+
+def _need(arr, nbytes: int, what: str) -> None:
+    """Raise before a launch when a device buffer is shorter than the kernel
+    reads or writes (the C ABI takes bare pointers and cannot check)."""
+    if arr is None or arr.ptr is None:
+        raise ValueError(f"{what}: buffer missing or freed")
+    if arr.nbytes < nbytes:
+        raise ValueError(f"{what}: {arr.nbytes} bytes, need at least {nbytes}")
 
 
 def _ptr(a: np.ndarray | None) -> int | None:
@@ -509,6 +519,9 @@ class Context:
 
     def set_list_device(self, n: int, nop: DeviceArray, ptr: DeviceArray, lst: DeviceArray,
                         n_entries: int, kind: int) -> None:
+        _need(nop, 4 * n, "number_of_partners")
+        _need(ptr, 4 * n, "pointer")
+        _need(lst, 4 * n_entries, "sorted_list")
         _check(lib().ljgpu_set_list_csr(self.handle, n, nop.ptr, ptr.ptr, lst.ptr, n_entries,
                                         kind, MEM_DEVICE))
 
@@ -521,10 +534,13 @@ class Context:
                    search_length: float = SEARCH_LENGTH) -> None:
         """Build the Verlet list on the device from device AoS positions
         (the reference's makepair + sortpair, exact pair set and order)."""
+        _need(q, 32 * n, "q (AoS double4)")
         _check(lib().ljgpu_build_list(self.handle, q.ptr, n, box, search_length, kind))
 
     def build_list_soa(self, q3: list, n: int, box: float = 0.0, kind: int = LIST_HALF,
                        search_length: float = SEARCH_LENGTH) -> None:
+        for c in range(3):
+            _need(q3[c], 8 * n, f"q[{c}] (SoA)")
         _check(lib().ljgpu_build_list_soa(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, n, box,
                                           search_length, kind))
 
@@ -574,6 +590,7 @@ class Context:
                  reps: int = 20) -> tuple[dict, list[dict]]:
         """Time every candidate kernel configuration on the installed list,
         install the fastest; returns (best, table)."""
+        _need(q, 32 * self._particles(), "q (AoS double4)")
         best = TuneEntry()
         _check(lib().ljgpu_autotune(self.handle, q.ptr, box, cl2, precision, accum, reps,
                                     C.byref(best)))
@@ -586,6 +603,12 @@ class Context:
     def set_graphs(self, enable: bool) -> None:
         _check(lib().ljgpu_set_graphs(self.handle, 1 if enable else 0))
 
+    def graph_stats(self) -> tuple[int, int]:
+        """(graphs captured over the context's life, graphs cached now)."""
+        cap, cached = _i64(0), _i32(0)
+        _check(lib().ljgpu_graph_stats(self.handle, C.byref(cap), C.byref(cached)))
+        return cap.value, cached.value
+
     def energy(self, q: DeviceArray, p: DeviceArray, box: float = 0.0,
                cl2: float = CL2) -> tuple[float, float]:
         """(potential, kinetic) over the installed list (unshifted potential)."""
@@ -594,6 +617,7 @@ class Context:
     def energy_pairs(self, q: DeviceArray, p: DeviceArray, box: float = 0.0,
                      cl2: float = CL2) -> tuple[float, float, int]:
         """(potential, kinetic, pairs inside the cutoff)."""
+        self._check_aos(q, p)
         u, k, c = _d(0.0), _d(0.0), _i64(0)
         _check(lib().ljgpu_energy(self.handle, q.ptr, p.ptr, box, cl2, C.byref(u), C.byref(k),
                                   C.byref(c)))
@@ -607,6 +631,8 @@ class Context:
         and the energy samples (rows: step, U, K, U + K). In a periodic box
         no list stays installed afterwards (the run's lists index its
         internal, spatially sorted copies)."""
+        _need(q, 32 * q.shape[0], "q (AoS double4)")
+        _need(p, 32 * q.shape[0], "p (AoS double4)")
         cfg = MDConfig(dt, cutoff, skin, rebuild_every, precision, accum, sample_every)
         st = MDStats()
         cap = max_samples if max_samples > 0 else (steps // sample_every + 2 if sample_every else 2)
@@ -634,27 +660,50 @@ class Context:
         return CSR(nop, ptr, lst[:total].copy(), kind)
 
     # forces
+    def _particles(self) -> int:
+        return int(self.list_info().particle_number)
+
+    def _check_aos(self, q: DeviceArray, p: DeviceArray) -> None:
+        n = self._particles()
+        _need(q, 32 * n, "q (AoS double4)")
+        _need(p, 32 * n, "p (AoS double4)")
+
+    def _check_soa(self, q3: list, p3: list) -> None:
+        n = self._particles()
+        for c in range(3):
+            _need(q3[c], 8 * n, f"q[{c}] (SoA)")
+            _need(p3[c], 8 * n, f"p[{c}] (SoA)")
+
     def force_aos_f64(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
                       box: float = 0.0, steps: int = 1) -> None:
+        self._check_aos(q, p)
         _check(lib().ljgpu_force_aos_f64(self.handle, q.ptr, p.ptr, dt, cl2, box, steps))
 
     def force_soa_f64(self, q3: list, p3: list, dt: float, cl2: float = CL2, box: float = 0.0,
                       steps: int = 1) -> None:
+        self._check_soa(q3, p3)
         _check(lib().ljgpu_force_soa_f64(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, p3[0].ptr,
                                          p3[1].ptr, p3[2].ptr, dt, cl2, box, steps))
 
     def force_aos_mixed(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
                         box: float = 0.0, steps: int = 1, accum: int = ACCUM_F64) -> None:
+        self._check_aos(q, p)
         _check(lib().ljgpu_force_aos_mixed(self.handle, q.ptr, p.ptr, dt, cl2, box, steps, accum))
 
     def force_soa_mixed(self, q3: list, p3: list, dt: float, cl2: float = CL2, box: float = 0.0,
                         steps: int = 1, accum: int = ACCUM_F64) -> None:
+        self._check_soa(q3, p3)
         _check(lib().ljgpu_force_soa_mixed(self.handle, q3[0].ptr, q3[1].ptr, q3[2].ptr, p3[0].ptr,
                                            p3[1].ptr, p3[2].ptr, dt, cl2, box, steps, accum))
 
     def force_kernel_csr_f64(self, q: DeviceArray, p: DeviceArray, n: int, dt: float, cl2: float,
                              sorted_list: DeviceArray, nop: DeviceArray, ptr: DeviceArray,
                              n_entries: int, kind: int) -> None:
+        _need(q, 32 * n, "q (AoS double4)")
+        _need(p, 32 * n, "p (AoS double4)")
+        _need(nop, 4 * n, "number_of_partners")
+        _need(ptr, 4 * n, "pointer")
+        _need(sorted_list, 4 * n_entries, "sorted_list")
         _check(lib().ljgpu_force_kernel_csr_f64(self.handle, q.ptr, p.ptr, n, dt, cl2,
                                                 sorted_list.ptr, nop.ptr, ptr.ptr, n_entries,
                                                 kind))

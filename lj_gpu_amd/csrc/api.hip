@@ -95,6 +95,14 @@ int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
                        ctx->work, ctx->tiles));
     if (ctx->tiles.tile_particles > 0) break;
   }
+  // The slot-order copies that calls of kSlotMinSteps or more steps run over
+  // are reserved with the tiles, so a force call never allocates.
+  if (ctx->tiles.blocks > 0 && ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL &&
+      ctx->tiles.staged > 0) {
+    const size_t bytes = sizeof(double) * 4 * static_cast<size_t>(ctx->eval.n);
+    TRY(dev_reserve(ctx->slot_q, bytes, ctx->stream));
+    TRY(dev_reserve(ctx->slot_p, bytes, ctx->stream));
+  }
   ctx->tiles_for = want;
   ctx->tiles_ordered = spatial && q;
   return LJGPU_OK;
@@ -317,9 +325,11 @@ int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum) {
 }
 
 void drop_graph(ljgpu_ctx* ctx) {
-  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-  ctx->graph_exec = nullptr;
-  ctx->graph_steps = 0;
+  // Replays already enqueued finish before their graphs are destroyed.
+  if (!ctx->graphs.empty() && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (GraphEntry& g : ctx->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
 }
 
 int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out) {
@@ -337,6 +347,39 @@ int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* ou
   const hipError_t e2 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e2 != hipSuccess) return hip_fail(e2, "hipGraphInstantiate");
+  return LJGPU_OK;
+}
+
+int cached_graph(ljgpu_ctx* ctx, const StepKey& key, const std::function<int()>& body,
+                 hipGraphExec_t* out) {
+  ctx->graph_clock++;
+  for (GraphEntry& g : ctx->graphs) {
+    if (std::memcmp(&g.key, &key, sizeof(key)) == 0) {
+      g.last_use = ctx->graph_clock;
+      *out = g.exec;
+      return LJGPU_OK;
+    }
+  }
+  hipGraphExec_t exec = nullptr;
+  TRY(capture(ctx, body, &exec));
+  ctx->graph_captures++;
+  if (static_cast<int>(ctx->graphs.size()) >= kGraphCacheSize) {
+    auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                [](const GraphEntry& a, const GraphEntry& b) {
+                                  return a.last_use < b.last_use;
+                                });
+    // Work already enqueued from the evicted graph must finish before it
+    // is destroyed.
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    (void)hipGraphExecDestroy(lru->exec);
+    ctx->graphs.erase(lru);
+  }
+  GraphEntry g;
+  g.key = key;
+  g.exec = exec;
+  g.last_use = ctx->graph_clock;
+  ctx->graphs.push_back(g);
+  *out = exec;
   return LJGPU_OK;
 }
 
@@ -439,9 +482,8 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& caller, int32_t steps, bool mixed
   }
   int32_t done = 0;
   if (ctx->use_graphs && steps >= kGraphMinSteps) {
-    // Repeated evaluations run as replays of one captured graph of up to
-    // kGraphChunk launches, re-captured only when an argument changes.
-    const int32_t chunk = std::min(steps, kGraphChunk);
+    // Repeated evaluations run as replays of captured graphs, longest chunk
+    // first (kGraphChunks); each graph is captured once per argument set.
     StepKey key;
     std::memset(&key, 0, sizeof(key));
     key.fa = fa;
@@ -449,20 +491,17 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& caller, int32_t steps, bool mixed
     key.lc = lc;
     key.mixed = mixed;
     key.accum = accum;
-    key.steps = chunk;
-    if (!ctx->graph_exec || std::memcmp(&key, &ctx->graph_key, sizeof(key)) != 0) {
-      drop_graph(ctx);
+    for (const int32_t chunk : kGraphChunks) {
+      if (steps - done < chunk) continue;
+      key.steps = chunk;
       hipGraphExec_t exec = nullptr;
-      TRY(capture(ctx, [&]() {
+      TRY(cached_graph(ctx, key, [&]() {
         for (int32_t s = 0; s < chunk; s++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
         return LJGPU_OK;
       }, &exec));
-      ctx->graph_exec = exec;
-      ctx->graph_key = key;
-      ctx->graph_steps = chunk;
+      for (; done + chunk <= steps; done += chunk)
+        HIP_TRY(hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
     }
-    for (; done + chunk <= steps; done += chunk)
-      HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream), "hipGraphLaunch");
   }
   for (; done < steps; done++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
   if (slots) TRY(slot_scatter(ctx, fa, caller));

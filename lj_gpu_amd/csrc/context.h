@@ -14,10 +14,17 @@
 
 namespace ljgpu {
 
-// Graph replay of repeated force evaluations: calls with at least
-// kGraphMinSteps steps run as replays of one captured graph of up to
-// kGraphChunk launches.
+// Graph replay of repeated force evaluations. A force call of `steps`
+// launches runs as replays of captured graphs of kGraphChunks[c] launches,
+// taken greedily from the longest (so 20 steps = 16 + 4), then fewer than
+// kGraphMinSteps single launches. A graph is captured once per set of kernel
+// arguments and chunk length and kept in a small cache (kGraphCacheSize
+// entries, least recently used evicted), so calls that alternate step counts,
+// precisions or arrays replay their graphs instead of re-capturing them.
 constexpr int32_t kGraphMinSteps = 4;
+constexpr int32_t kGraphChunks[3] = {64, 16, 4};
+constexpr int kGraphCacheSize = 12;
+// Longest run of (drift, kick) pairs the MD loop captures as one graph.
 constexpr int32_t kGraphChunk = 100;
 // Calls with at least kSlotMinSteps steps over spatially ordered tiles run
 // over copies of q and p in slot order (two gathers and a scatter per call).
@@ -32,6 +39,12 @@ struct StepKey {
   int32_t mixed;
   int32_t accum;
   int32_t steps;
+};
+
+struct GraphEntry {
+  StepKey key;
+  hipGraphExec_t exec = nullptr;
+  uint64_t last_use = 0;
 };
 
 }  // namespace ljgpu
@@ -73,11 +86,11 @@ struct ljgpu_ctx {
 
   ljgpu::ListWork work;
 
-  // Captured force loop (see StepKey).
+  // Captured force loops (see StepKey), at most kGraphCacheSize of them.
   bool use_graphs = true;
-  hipGraphExec_t graph_exec = nullptr;
-  ljgpu::StepKey graph_key{};
-  int32_t graph_steps = 0;
+  std::vector<ljgpu::GraphEntry> graphs;
+  uint64_t graph_clock = 0;
+  int64_t graph_captures = 0;  // graphs captured over the context's life
 
   // Batched replicas: particle offsets (n_replicas + 1) and boxes on the
   // host, (box, 1/box) per replica on the device; empty for one system.
@@ -144,5 +157,8 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
 // Record the launches `body` enqueues on the context stream into an
 // executable graph.
 int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out);
+// The context's cached graph for `key`, captured from `body` on a miss.
+int cached_graph(ljgpu_ctx* ctx, const StepKey& key, const std::function<int()>& body,
+                 hipGraphExec_t* out);
 
 }  // namespace ljgpu

@@ -591,4 +591,11 @@ int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable) {
   return LJGPU_OK;
 }
 
+int ljgpu_graph_stats(ljgpu_ctx* ctx, int64_t* captures, int32_t* cached) {
+  if (!ctx || !captures || !cached) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  *captures = ctx->graph_captures;
+  *cached = static_cast<int32_t>(ctx->graphs.size());
+  return LJGPU_OK;
+}
+
 }  // extern "C"

@@ -209,6 +209,52 @@ def time_force_sorted_aos(csr: CSR, q: np.ndarray, dt: float, cl2: float = CL2,
                                            box, steps)
 
 
+# ---------------------------------------------------------------------------
+# Timed CPU baselines (oracle/liblj_cpubase.so, built from cpu_baseline.cpp)
+# ---------------------------------------------------------------------------
+BASELINE_PATH = os.path.join(HERE, "liblj_cpubase.so")
+BASELINE_SORTED, BASELINE_MAT_TRANSPOSE, BASELINE_FULL_OMP = 0, 1, 2
+_cb = None
+
+
+def baseline_lib() -> C.CDLL:
+    global _cb
+    if _cb is None:
+        if not os.path.exists(BASELINE_PATH):
+            raise RuntimeError(f"CPU baseline library missing: {BASELINE_PATH} "
+                               "(run `make -C oracle oracle`)")
+        L = C.CDLL(BASELINE_PATH)
+        L.cb_run.argtypes = [C.c_int, C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, C.c_double,
+                             C.c_double, C.c_double, C.c_int32]
+        L.cb_run.restype = C.c_double
+        L.cb_threads.restype = C.c_int
+        L.cb_build_info.restype = C.c_char_p
+        _cb = L
+    return _cb
+
+
+def aligned_zeros(shape, align: int = 64) -> np.ndarray:
+    """float64 zeros whose data starts on an `align`-byte boundary."""
+    n = int(np.prod(shape))
+    raw = np.zeros(n + align // 8, dtype=np.float64)
+    off = (-raw.ctypes.data % align) // 8
+    return raw[off: off + n].reshape(shape)
+
+
+def run_baseline(kind: int, csr: CSR, q: np.ndarray, p: np.ndarray, dt: float,
+                 cl2: float = CL2, box: float = 0.0, steps: int = 1) -> float:
+    """`steps` evaluations of CPU baseline `kind` on p (in place, AoS (N, 4),
+    32-byte aligned); returns wall seconds."""
+    assert q.ctypes.data % 32 == 0 and p.ctypes.data % 32 == 0
+    nop = csr.number_of_partners if csr.n else np.zeros(1, np.int32)
+    ptr = csr.pointer if csr.n else np.zeros(1, np.int32)
+    t = baseline_lib().cb_run(kind, csr.n, nop, ptr, _lst(csr), q.reshape(-1), p.reshape(-1),
+                              dt, cl2, box, steps)
+    if t < 0:
+        raise ValueError(f"unknown baseline kind {kind}")
+    return t
+
+
 def format_reference_output(p: np.ndarray) -> str:
     """The reference's print_results(): first and last 5 momenta at %.10f
     (cpu_ref/force_aos.cpp:622-630)."""

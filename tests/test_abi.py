@@ -56,3 +56,18 @@ def test_no_device_is_an_error_not_a_crash():
     assert h.value is None
     with pytest.raises(G.LJGPUError):
         G.Context(0)
+
+
+def test_buffer_size_checks_reject_short_buffers():
+    """The Python mirror checks device buffer sizes before calling into the
+    library (the C ABI has bare pointers); host-side only."""
+    class Fake:
+        def __init__(self, nbytes, ptr=1):
+            self.nbytes, self.ptr = nbytes, ptr
+    G._need(Fake(32 * 10), 32 * 10, "q")
+    with pytest.raises(ValueError):
+        G._need(Fake(32 * 10 - 8), 32 * 10, "q")
+    with pytest.raises(ValueError):
+        G._need(Fake(64, ptr=None), 32, "q")
+    with pytest.raises(ValueError):
+        G._need(None, 0, "q")

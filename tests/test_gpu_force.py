@@ -77,12 +77,11 @@ def run_soa(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=16, mixed=None,
 # FP64 parity on the periodic workloads
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
-@pytest.mark.parametrize("lanes", [1, 4, 8, 16, 32, 64])
+# Lanes 1 selects the ELL kernel, which is never tiled.
+@pytest.mark.parametrize("lanes,tile", [(1, 0)] + [(l, t) for l in (4, 8, 16, 32, 64)
+                                                   for t in (0, 64)])
 @pytest.mark.parametrize("unroll", [1, 2, 4])
-@pytest.mark.parametrize("tile", [0, 64])
 def test_f64_full_list_aos(ctx, k, rho, lanes, unroll, tile):
-    if lanes == 1 and tile:
-        pytest.skip("the ELL kernel is never tiled")
     q, box, _, full = periodic(k, rho)
     p = run_aos(ctx, q, full, box, lanes=lanes, unroll=unroll, tile=tile)
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
@@ -170,7 +169,7 @@ def test_f64_half_list_aos(ctx, k, rho, mode, lanes, unroll):
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
 
-@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0)])
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
 @pytest.mark.parametrize("kind", ["full", "half_atomic", "half_expand"])
 def test_f64_soa(ctx, k, rho, kind):
     q, box, half, full = periodic(k, rho)
@@ -408,3 +407,64 @@ def test_slot_order_copies_are_bitwise_per_step(ctx, layout, prec, graphs):
     np.testing.assert_array_equal(res[0], res[1])
     if prec == "f64":
         assert normwise(res[0][:, :3], oracle_momenta(12, 1.0, steps=20)) < FP64_TOL
+
+
+def test_graph_cache_reuses_graphs_across_step_counts_and_precisions(ctx):
+    """Calls that alternate step counts and precisions replay cached graphs
+    (64/16/4-launch chunks, captured once per argument set) and give the
+    same momenta, bit for bit, as individual launches."""
+    q, box, _, full = periodic(12, 1.0)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    plan = [("f64", 20), ("mixed", 20), ("f64", 50), ("f64", 20), ("mixed", 5), ("f64", 100),
+            ("mixed", 20), ("f64", 50), ("f64", 3)]
+    res = {}
+    for graphs in (False, True):
+        ctx.set_graphs(graphs)
+        dq, dp = ctx.array(q), ctx.zeros(q.shape)
+        counts = []
+        for prec, steps in plan:
+            if prec == "f64":
+                ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, steps)
+            else:
+                ctx.force_aos_mixed(dq, dp, PBC_DT, G.CL2, box, steps, G.ACCUM_F64)
+            counts.append(ctx.graph_stats()[0])
+        ctx.sync()
+        res[graphs] = dp.download()
+        if graphs:
+            # 20 = 16 + 4, 50 = 16 * 3 + 2, 100 = 64 + 16 * 2 + 4, 5 = 4 + 1:
+            # the repeated (precision, chunk) pairs capture nothing new.
+            new = np.diff([counts[0] - 2] + counts)
+            assert list(new) == [2, 2, 0, 0, 1, 1, 0, 0, 0], counts
+            assert ctx.graph_stats()[1] == 6
+        dq.free()
+        dp.free()
+    ctx.set_graphs(True)
+    np.testing.assert_array_equal(res[True], res[False])
+
+
+def test_short_buffers_are_rejected_before_any_launch(ctx):
+    q, box, _, full = periodic(6, 1.0)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_list(full)
+    dq = ctx.array(q)
+    short = ctx.zeros((q.shape[0] - 1, 4))
+    with pytest.raises(ValueError):
+        ctx.force_aos_f64(dq, short, PBC_DT, G.CL2, box, 1)
+    with pytest.raises(ValueError):
+        ctx.force_aos_mixed(short, dq, PBC_DT, G.CL2, box, 1)
+    q3 = [ctx.zeros(q.shape[0]) for _ in range(3)]
+    p3 = [ctx.zeros(q.shape[0]) for _ in range(2)] + [ctx.zeros(q.shape[0] - 1)]
+    with pytest.raises(ValueError):
+        ctx.force_soa_f64(q3, p3, PBC_DT, G.CL2, box, 1)
+    d_nop = ctx.array(full.number_of_partners)
+    d_ptr = ctx.array(full.pointer)
+    d_lst = ctx.array(full.sorted_list[:-1].copy())
+    with pytest.raises(ValueError):
+        ctx.set_list_device(full.n, d_nop, d_ptr, d_lst, full.n_entries, G.LIST_FULL)
+    with pytest.raises(ValueError):
+        ctx.force_kernel_csr_f64(dq, dq, full.n, 0.01, G.CL2, d_lst, d_nop, d_ptr,
+                                 full.n_entries, G.LIST_FULL)
