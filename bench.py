@@ -320,6 +320,8 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
     ap.add_argument("--unroll", type=int, default=0, help="entries per lane per batch (0: default)")
     ap.add_argument("--tile", type=int, default=-1, help="particles per LDS tile (-1: default)")
+    ap.add_argument("--chains", type=int, default=0,
+                    help="concurrent launch chains over the tiles (0: library default)")
     ap.add_argument("--tile-order", choices=["spatial", "index"], default="spatial",
                     help="order of the particles in LDS tiles")
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
@@ -375,6 +377,8 @@ def main() -> int:
     if args.tile >= 0:
         ctx.set_tiling(args.tile)
     ctx.set_tile_order(G.TILE_ORDER_INDEX if args.tile_order == "index" else G.TILE_ORDER_SPATIAL)
+    if args.chains:
+        ctx.set_chains(args.chains)
     ctx.set_list(full)
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)

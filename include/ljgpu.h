@@ -376,6 +376,17 @@ int ljgpu_maxwell_boltzmann(int32_t particle_number, double temperature, uint32_
  * recently used evicted). */
 int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable);
 
+/* Concurrent chains of a many-step call over LDS tiles (1 to 4, default 2).
+ * With c > 1, a call of 4 or more steps over at least 64 * c tiles splits
+ * the tiles into c contiguous parts and runs each part's steps as its own
+ * chain of launches on its own stream of the context, forked from and joined
+ * back into the context stream within the call (launched directly, not
+ * replayed from graphs). The parts update disjoint particles, so the momenta
+ * are bitwise those of one chain; one chain's launch ramp and tail overlap
+ * the others' work. No counterpart in the reference (which launches on the
+ * default stream). */
+int ljgpu_set_chains(ljgpu_ctx* ctx, int chains);
+
 /* Graph cache state: graphs captured over the context's life and graphs
  * cached now. No counterpart in the reference (which has no graphs); lets a
  * caller check that repeated calls replay instead of re-capturing. */

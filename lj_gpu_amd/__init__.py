@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
         "ljgpu_maxwell_boltzmann": [_i32, _d, C.c_uint32, _vp],
         "ljgpu_set_graphs": [_vp, C.c_int],
         "ljgpu_graph_stats": [_vp, C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_set_chains": [_vp, C.c_int],
         "ljgpu_pair_dat_write": [C.c_char_p, _i32, _vp, _vp, _vp, _i64, _i32, _i64],
         "ljgpu_pair_dat_read": [C.c_char_p, _i32, _i32, _i64, _vp, _vp, _vp, _i64,
                                 C.POINTER(_i64)],
@@ -602,6 +603,10 @@ class Context:
 
     def set_graphs(self, enable: bool) -> None:
         _check(lib().ljgpu_set_graphs(self.handle, 1 if enable else 0))
+
+    def set_chains(self, chains: int) -> None:
+        """Concurrent launch chains of a many-step call over tiles (1 or 2)."""
+        _check(lib().ljgpu_set_chains(self.handle, chains))
 
     def graph_stats(self) -> tuple[int, int]:
         """(graphs captured over the context's life, graphs cached now)."""

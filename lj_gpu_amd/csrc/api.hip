@@ -309,11 +309,16 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
   return LJGPU_OK;
 }
 
+int launch_one_on(hipStream_t stream, const ListView& lv, const LaunchConfig& lc,
+                  const ForceArgs& fa, bool mixed, int accum) {
+  return !mixed ? launch_force_f64(stream, lv, fa, lc)
+         : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(stream, lv, fa, lc)
+                                          : launch_force_mixed_f64acc(stream, lv, fa, lc);
+}
+
 int launch_one(ljgpu_ctx* ctx, const ListView& lv, const LaunchConfig& lc, const ForceArgs& fa,
                bool mixed, int accum) {
-  return !mixed ? launch_force_f64(ctx->stream, lv, fa, lc)
-         : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(ctx->stream, lv, fa, lc)
-                                          : launch_force_mixed_f64acc(ctx->stream, lv, fa, lc);
+  return launch_one_on(ctx->stream, lv, lc, fa, mixed, accum);
 }
 
 int enqueue_force(ljgpu_ctx* ctx, const ForceArgs& fa, bool mixed, int accum) {
@@ -350,11 +355,55 @@ int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* ou
   return LJGPU_OK;
 }
 
+StepKey step_key(const ForceArgs& fa, const ListView& lv, const LaunchConfig& lc, bool mixed,
+                 int accum, int32_t steps, int32_t chains) {
+  StepKey k;
+  k.add(fa.soa);
+  for (int c = 0; c < 3; c++) {
+    k.add(fa.q[c]);
+    k.add(fa.p[c]);
+  }
+  k.add(fa.dt);
+  k.add(fa.cl2);
+  k.add(fa.box);
+  k.add(lv.n);
+  k.add(lv.nop);
+  k.add(lv.ptr);
+  k.add(lv.list);
+  k.add(lv.n_entries);
+  k.add(lv.ell_width);
+  k.add(lv.tile_blocks);
+  k.add(lv.tile_first);
+  k.add(lv.tile_particles);
+  k.add(lv.tile_lds_bytes);
+  k.add(lv.tile_max_stage);
+  k.add(lv.tile_desc);
+  k.add(lv.tile_stage);
+  k.add(lv.tile_local);
+  k.add(lv.tile_member);
+  k.add(lv.tile_slot_ptr);
+  k.add(lv.tile_stage_slot);
+  k.add(lv.replica_box);
+  k.add(lc.lanes);
+  k.add(lc.unroll);
+  k.add(lc.block);
+  k.add(lc.tile_offsets_lds);
+  k.add(lc.tile_resident);
+  k.add(lc.cus);
+  k.add(lc.half_atomic);
+  k.add(lc.clamp);
+  k.add(mixed);
+  k.add(accum);
+  k.add(steps);
+  k.add(chains);
+  return k;
+}
+
 int cached_graph(ljgpu_ctx* ctx, const StepKey& key, const std::function<int()>& body,
                  hipGraphExec_t* out) {
   ctx->graph_clock++;
   for (GraphEntry& g : ctx->graphs) {
-    if (std::memcmp(&g.key, &key, sizeof(key)) == 0) {
+    if (g.key == key) {
       g.last_use = ctx->graph_clock;
       *out = g.exec;
       return LJGPU_OK;
@@ -480,30 +529,52 @@ int run_steps(ljgpu_ctx* ctx, const ForceArgs& caller, int32_t steps, bool mixed
     lv.tile_member = nullptr;
     lv.tile_stage = lv.tile_stage_slot;
   }
-  int32_t done = 0;
-  if (ctx->use_graphs && steps >= kGraphMinSteps) {
-    // Repeated evaluations run as replays of captured graphs, longest chunk
-    // first (kGraphChunks); each graph is captured once per argument set.
-    StepKey key;
-    std::memset(&key, 0, sizeof(key));
-    key.fa = fa;
-    key.lv = lv;
-    key.lc = lc;
-    key.mixed = mixed;
-    key.accum = accum;
-    for (const int32_t chunk : kGraphChunks) {
-      if (steps - done < chunk) continue;
-      key.steps = chunk;
-      hipGraphExec_t exec = nullptr;
-      TRY(cached_graph(ctx, key, [&]() {
-        for (int32_t s = 0; s < chunk; s++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
-        return LJGPU_OK;
-      }, &exec));
-      for (; done + chunk <= steps; done += chunk)
-        HIP_TRY(hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+  // Over enough tiles, the tiles in `chains` contiguous parts, each a chain
+  // of launches on its own stream, forked from and joined back into the
+  // context stream (see kMaxChains).
+  const int chains =
+      (lv.tile_blocks >= kMinChainTiles * ctx->chains && steps >= kGraphMinSteps) ? ctx->chains : 1;
+  if (chains > 1) {
+    HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream), "hipEventRecord");
+    ListView part[kMaxChains];
+    for (int c = 0; c < chains; c++) {
+      part[c] = lv;
+      const int32_t a = static_cast<int32_t>(int64_t{lv.tile_blocks} * c / chains);
+      const int32_t b = static_cast<int32_t>(int64_t{lv.tile_blocks} * (c + 1) / chains);
+      part[c].tile_first = a;
+      part[c].tile_blocks = b - a;
+      if (c > 0)
+        HIP_TRY(hipStreamWaitEvent(ctx->chain_stream[c - 1], ctx->ev_fork, 0),
+                "hipStreamWaitEvent");
     }
+    for (int32_t s = 0; s < steps; s++)
+      for (int c = 0; c < chains; c++)
+        TRY(launch_one_on(c == 0 ? ctx->stream : ctx->chain_stream[c - 1], part[c], lc, fa, mixed,
+                          accum));
+    for (int c = 1; c < chains; c++) {
+      HIP_TRY(hipEventRecord(ctx->ev_join[c - 1], ctx->chain_stream[c - 1]), "hipEventRecord");
+      HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join[c - 1], 0), "hipStreamWaitEvent");
+    }
+  } else {
+    int32_t done = 0;
+    if (ctx->use_graphs && steps >= kGraphMinSteps) {
+      // Repeated evaluations run as replays of captured graphs, longest
+      // chunk first (kGraphChunks); each graph is captured once per argument
+      // set.
+      for (const int32_t chunk : kGraphChunks) {
+        if (steps - done < chunk) continue;
+        const StepKey key = step_key(fa, lv, lc, mixed, accum, chunk, 1);
+        hipGraphExec_t exec = nullptr;
+        TRY(cached_graph(ctx, key, [&]() {
+          for (int32_t s = 0; s < chunk; s++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
+          return LJGPU_OK;
+        }, &exec));
+        for (; done + chunk <= steps; done += chunk)
+          HIP_TRY(hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+      }
+    }
+    for (; done < steps; done++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
   }
-  for (; done < steps; done++) TRY(launch_one(ctx, lv, lc, fa, mixed, accum));
   if (slots) TRY(slot_scatter(ctx, fa, caller));
   return LJGPU_OK;
 }
@@ -539,6 +610,11 @@ int ljgpu_create(int device, ljgpu_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev_start);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stop);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  for (int c = 0; c + 1 < kMaxChains && e == hipSuccess; c++) {
+    e = hipStreamCreateWithFlags(&ctx->chain_stream[c], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join[c], hipEventDisableTiming);
+  }
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
@@ -558,6 +634,12 @@ int ljgpu_destroy(ljgpu_ctx* ctx) {
   dev_release(ctx->rep_box_dev);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  for (int c = 0; c + 1 < kMaxChains; c++) {
+    if (ctx->ev_join[c]) (void)hipEventDestroy(ctx->ev_join[c]);
+    if (ctx->chain_stream[c]) (void)hipStreamSynchronize(ctx->chain_stream[c]);
+    if (ctx->chain_stream[c]) (void)hipStreamDestroy(ctx->chain_stream[c]);
+  }
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LJGPU_OK;
@@ -678,8 +760,9 @@ int ljgpu_set_half_mode(ljgpu_ctx* ctx, int half_mode) {
 
 int ljgpu_set_kernel_config(ljgpu_ctx* ctx, int lanes, int block, int unroll) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
-  if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
-    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "lanes_per_particle must be 1,4,8,16,32 or 64");
+  if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16 &&
+      lanes != 32 && lanes != 64)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "lanes_per_particle must be 1,2,4,8,16,32 or 64");
   if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "block size must be a multiple of 64 in [64, 1024]");
   if (unroll != 0 && unroll != 1 && unroll != 2 && unroll != 4)

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <functional>
 #include <string>
 #include <vector>
@@ -26,20 +27,38 @@ constexpr int32_t kGraphChunks[3] = {64, 16, 4};
 constexpr int kGraphCacheSize = 12;
 // Longest run of (drift, kick) pairs the MD loop captures as one graph.
 constexpr int32_t kGraphChunk = 100;
+// A call of kGraphMinSteps or more steps over LDS tiles splits the tiles into
+// `chains` (1 to kMaxChains) contiguous parts and runs the steps of each part
+// as its own chain of launches on its own stream. The parts update disjoint
+// particles, so the chains need no ordering between them and every particle
+// receives the same additions in the same order; one chain's launch ramp and
+// tail overlap the others' work. Chained calls launch directly on the
+// streams (a captured graph of the same fork and join ran its branches with
+// less overlap on MI355X); single-chain calls replay graphs.
+constexpr int kMaxChains = 4;
+constexpr int32_t kMinChainTiles = 64;  // fewer tiles: one chain
 // Calls with at least kSlotMinSteps steps over spatially ordered tiles run
 // over copies of q and p in slot order (two gathers and a scatter per call).
 constexpr int32_t kSlotMinSteps = 16;
 
-// Every argument of a captured force loop; a graph is replayed only for an
-// identical key (compared bytewise, so it is zero-filled before use).
+// Every argument of a captured force loop, written field by field into a
+// byte string (so padding inside the argument structs never enters it); a
+// graph is replayed only for an identical key.
 struct StepKey {
-  ForceArgs fa;
-  ListView lv;
-  LaunchConfig lc;
-  int32_t mixed;
-  int32_t accum;
-  int32_t steps;
+  unsigned char bytes[384];
+  size_t size = 0;
+  template <class T>
+  void add(const T& v) {
+    static_assert(sizeof(T) <= 8, "scalar fields only");
+    std::memcpy(bytes + size, &v, sizeof(T));
+    size += sizeof(T);
+  }
+  bool operator==(const StepKey& o) const {
+    return size == o.size && std::memcmp(bytes, o.bytes, size) == 0;
+  }
 };
+StepKey step_key(const ForceArgs& fa, const ListView& lv, const LaunchConfig& lc, bool mixed,
+                 int accum, int32_t steps, int32_t chains);
 
 struct GraphEntry {
   StepKey key;
@@ -54,6 +73,13 @@ struct ljgpu_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   bool timer_running = false;
+  // Further chains of a many-step call over tiles (see kMaxChains): their
+  // own streams (default priority), forked from and joined back into
+  // `stream`.
+  hipStream_t chain_stream[ljgpu::kMaxChains - 1] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[ljgpu::kMaxChains - 1] = {};
+  int chains = 2;
 
   // List as given by the caller (or built by ljgpu_build_list).
   int32_t n = 0;

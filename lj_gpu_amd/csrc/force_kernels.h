@@ -347,9 +347,9 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __
 // offsets from global memory instead (less LDS per workgroup, so more
 // resident workgroups per CU).
 //
-// Tiles are assigned XCD-aware: the workgroups on XCD x (blockIdx % 8 == x)
-// walk the contiguous tile range [x*per, (x+1)*per) with a stride of
-// gridDim/8. With one workgroup per tile each does a single tile; with a
+// A launch covers tiles [first, first + nblocks). They are assigned
+// XCD-aware: the workgroups on XCD x (blockIdx % 8 == x) walk the contiguous
+// range [x*per, (x+1)*per) of the launch's tiles with a stride of gridDim/8. With one workgroup per tile each does a single tile; with a
 // smaller (persistent) grid each loops over several, so resident workgroups
 // drift out of phase and one's staging and memory stalls overlap another's
 // arithmetic.
@@ -479,8 +479,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ member,
        const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
        const int32_t* __restrict__ stage,
-       const uint16_t* __restrict__ local, int32_t nblocks, PairParams pp, double reach,
-       const double2* __restrict__ rep_box) {
+       const uint16_t* __restrict__ local, int32_t first, int32_t nblocks, PairParams pp,
+       double reach, const double2* __restrict__ rep_box) {
   // Dynamic LDS only (no static LDS, so its base is address 0 and constant
   // offsets fold into the LDS instructions): a 16-byte header holding the
   // frame-check word, then the tile's positions (and offsets).
@@ -495,7 +495,7 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   const int t_end = min(nblocks, (xcd + 1) * per);
   for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
     tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, member, slot_ptr, desc, stage,
-                                                 local, b, pp, reach, rep_box, lb, flag);
+                                                 local, first + b, pp, reach, rep_box, lb, flag);
     __syncthreads();  // the next tile overwrites this tile's LDS
   }
 }
@@ -550,7 +550,8 @@ int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& 
   hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(grid), dim3(threads),
                      lds_bytes, s, pos, mom, lv.nop, lv.tile_member, lv.tile_slot_ptr,
                      reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
-                     lv.tile_blocks, pp, reach, reinterpret_cast<const double2*>(lv.replica_box));
+                     lv.tile_first, lv.tile_blocks, pp, reach,
+                     reinterpret_cast<const double2*>(lv.replica_box));
   return launch_status("tiled force kernel launch");
 }
 
@@ -573,6 +574,7 @@ template <int PREC, class P, class M, bool PBC>
 int run_tile_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
                    const LaunchConfig& lc) {
   switch (lc.lanes) {
+    case 2: return run_tile_unroll<2, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
     case 4: return run_tile_unroll<4, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
     case 8: return run_tile_unroll<8, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
     case 16: return run_tile_unroll<16, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
@@ -580,7 +582,7 @@ int run_tile_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairPa
     case 64: return run_tile_unroll<64, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
     default:
       return set_error(LJGPU_ERR_INVALID_ARGUMENT,
-                       "lanes_per_particle must be 4, 8, 16, 32 or 64 for the tiled kernel");
+                       "lanes_per_particle must be 2, 4, 8, 16, 32 or 64 for the tiled kernel");
   }
 }
 
@@ -611,6 +613,7 @@ int run_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams&
     }
   }
   switch (lc.lanes) {
+    case 2: return run_unroll<2, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
     case 4: return run_unroll<4, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
     case 8: return run_unroll<8, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
     case 16: return run_unroll<16, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
@@ -618,7 +621,7 @@ int run_lanes(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams&
     case 64: return run_unroll<64, PREC, P, M, PBC, HALF, CLAMP>(s, pos, mom, lv, pp, lc);
     default:
       return set_error(LJGPU_ERR_INVALID_ARGUMENT,
-                       "lanes_per_particle must be 4, 8, 16, 32 or 64 for a CSR list");
+                       "lanes_per_particle must be 2, 4, 8, 16, 32 or 64 for a CSR list");
   }
 }
 

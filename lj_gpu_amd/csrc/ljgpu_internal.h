@@ -54,6 +54,7 @@ struct ListView {
   // entry offsets. `local` holds, per entry of the slot-ordered rows, the LDS
   // byte offset (24 * rank) of its j in the block's position area.
   int32_t tile_blocks = 0;
+  int32_t tile_first = 0;           // the launch covers blocks [tile_first, tile_first + tile_blocks)
   int32_t tile_particles = 0;       // max particles per block
   int32_t tile_lds_bytes = 0;       // max LDS bytes per block (positions + offsets)
   int32_t tile_max_stage = 0;       // max staged positions per block

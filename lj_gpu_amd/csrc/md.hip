@@ -482,7 +482,7 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
   if (steps > 0) TRY(kick(0.5 * dt));
   hipGraphExec_t seg_exec = nullptr;
   int32_t seg_len = 0;
-  StepKey seg_key{};
+  StepKey seg_key;
   auto drop_seg = [&]() {
     if (seg_exec) (void)hipGraphExecDestroy(seg_exec);
     seg_exec = nullptr;
@@ -543,14 +543,11 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t n, double box,
       continue;
     }
     fa.dt = dt;
-    StepKey key;
-    std::memset(&key, 0, sizeof(key));
-    TRY(force_view(ctx, fa, &key.lv, &key.lc));
-    key.fa = fa;
-    key.mixed = mixed;
-    key.accum = accum;
-    key.steps = len;
-    if (!seg_exec || seg_len != len || std::memcmp(&key, &seg_key, sizeof(key)) != 0) {
+    ListView klv;
+    LaunchConfig klc;
+    TRY(force_view(ctx, fa, &klv, &klc));
+    const StepKey key = step_key(fa, klv, klc, mixed, accum, len, 1);
+    if (!seg_exec || seg_len != len || !(key == seg_key)) {
       drop_seg();
       TRY(capture(ctx, [&]() {
         for (int32_t t = 0; t < len; t++) {
@@ -588,6 +585,14 @@ int ljgpu_set_graphs(ljgpu_ctx* ctx, int enable) {
   if (enable != 0 && enable != 1) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "enable must be 0 or 1");
   ctx->use_graphs = enable != 0;
   if (!ctx->use_graphs) drop_graph(ctx);
+  return LJGPU_OK;
+}
+
+int ljgpu_set_chains(ljgpu_ctx* ctx, int chains) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (chains < 1 || chains > kMaxChains)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "chains must be 1 to 4");
+  ctx->chains = chains;
   return LJGPU_OK;
 }
 

@@ -468,3 +468,35 @@ def test_short_buffers_are_rejected_before_any_launch(ctx):
     with pytest.raises(ValueError):
         ctx.force_kernel_csr_f64(dq, dq, full.n, 0.01, G.CL2, d_lst, d_nop, d_ptr,
                                  full.n_entries, G.LIST_FULL)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("prec", ["f64", G.ACCUM_F64])
+def test_chains_are_bitwise_one_chain(ctx, graphs, prec):
+    """A many-step call over tiles runs 2-4 parts of the tiles as concurrent
+    launch chains; they update disjoint particles, so the momenta equal those
+    of one chain bit for bit (spatial order, slot copies)."""
+    q, box, _, full = periodic(12, 1.0)
+    ctx.set_graphs(graphs)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(16)  # 432 tiles: both chains hold many
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    res = []
+    for chains in (1, 2, 3, 4):
+        ctx.set_chains(chains)
+        dq, dp = ctx.array(q), ctx.zeros(q.shape)
+        for steps in (37, 5, 64):
+            if prec == "f64":
+                ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, steps)
+            else:
+                ctx.force_aos_mixed(dq, dp, PBC_DT, G.CL2, box, steps, prec)
+        ctx.sync()
+        res.append(dp.download())
+    ctx.set_chains(2)
+    ctx.set_graphs(True)
+    ctx.set_tiling(64)
+    for r in res[1:]:
+        np.testing.assert_array_equal(res[0], r)
+    if prec == "f64":
+        assert normwise(res[1][:, :3], oracle_momenta(12, 1.0, steps=106)) < FP64_TOL
