@@ -471,6 +471,24 @@ int ljgpu_make_tiles_ordered(int32_t particle_number, const int32_t* number_of_p
                              int64_t stage_capacity, uint16_t* local, int32_t* n_blocks,
                              int64_t* n_stage, int32_t* max_stage);
 
+/* As ljgpu_make_tiles_ordered, with positions q (AoS double[N][4]; box > 0
+ * periodic, else open) that put each row of at most 256 entries in cutoff
+ * classes before the bank-class order: first the entries whose pair lies
+ * within the cutoff, r2 = (dx*dx + dy*dy) + dz*dz <= cl2 over the threshold
+ * minimum image of d = q_j - q_i, then the others, each part in list order
+ * and then bank-class ordered with its classes continuing from the part's
+ * first entry. The tiles a force evaluation derives from its positions take
+ * these classes; the force kernels then skip the force arithmetic of batches
+ * whose entries are all beyond the cutoff at that step. q = null is
+ * ljgpu_make_tiles_ordered. No reference counterpart: the host model of the
+ * device tiling. */
+int ljgpu_make_tiles_cut(int32_t particle_number, const int32_t* number_of_partners,
+                         const int32_t* pointer, const int32_t* sorted_list, int64_t n_entries,
+                         int32_t tile_particles, const int32_t* member, const double* q,
+                         double box, double cl2, int32_t* desc, int64_t desc_capacity,
+                         int32_t* stage, int64_t stage_capacity, uint16_t* local,
+                         int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage);
+
 /* ---- pair-list caches (host files, no device needed) ---------------------- */
 /* pair.dat, the binary cache of the CPU programs (savepair / loadpair,
  * cpu_ref/force_aos.cpp:603-620; cpu_ref/force_soa.cpp:360-376): int32

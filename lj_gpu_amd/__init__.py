@@ -163,6 +163,9 @@ def lib() -> C.CDLL:
                              C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
         "ljgpu_make_tiles_ordered": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _i64, _vp, _i64,
                                      _vp, C.POINTER(_i32), C.POINTER(_i64), C.POINTER(_i32)],
+        "ljgpu_make_tiles_cut": [_i32, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _d, _d, _vp, _i64,
+                                 _vp, _i64, _vp, C.POINTER(_i32), C.POINTER(_i64),
+                                 C.POINTER(_i32)],
         "ljgpu_set_tile_order": [_vp, C.c_int],
         "ljgpu_get_tile_order": [_vp, C.POINTER(C.c_int), _vp, _i64],
         "ljgpu_build_list": [_vp, _vp, _i32, _d, _d, C.c_int],
@@ -395,20 +398,24 @@ class Tiles:
     max_stage: int
 
 
-def make_tiles(csr: CSR, tile_particles: int = 64, member=None) -> Tiles:
+def make_tiles(csr: CSR, tile_particles: int = 64, member=None, q=None, box: float = 0.0,
+               cl2: float = CL2) -> Tiles:
     """Host model of the LDS tiling; member (one particle per slot) orders
-    the particles the blocks cover, None for index order."""
+    the particles the blocks cover, None for index order; positions q (N, 4)
+    put the rows in cutoff classes (entries within the cutoff cl2 first)."""
     nb, ns, ms = _i32(0), _i64(0), _i32(0)
     mem = None if member is None else np.ascontiguousarray(member, dtype=np.int32)
+    qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
     args = (csr.n, _ptr(csr.number_of_partners), _ptr(csr.pointer), _ptr(csr.sorted_list),
-            csr.n_entries, tile_particles, None if mem is None else _ptr(mem))
-    _check(lib().ljgpu_make_tiles_ordered(*args, None, 0, None, 0, None, C.byref(nb),
-                                          C.byref(ns), C.byref(ms)))
+            csr.n_entries, tile_particles, None if mem is None else _ptr(mem),
+            None if qq is None else _ptr(qq), box, cl2)
+    _check(lib().ljgpu_make_tiles_cut(*args, None, 0, None, 0, None, C.byref(nb),
+                                      C.byref(ns), C.byref(ms)))
     desc = np.zeros(max(8 * nb.value, 1), np.int32)
     stage = np.zeros(max(ns.value, 1), np.int32)
     local = np.zeros(max(csr.n_entries, 1), np.uint16)
-    _check(lib().ljgpu_make_tiles_ordered(*args, _ptr(desc), desc.size, _ptr(stage), stage.size,
-                                          _ptr(local), C.byref(nb), C.byref(ns), C.byref(ms)))
+    _check(lib().ljgpu_make_tiles_cut(*args, _ptr(desc), desc.size, _ptr(stage), stage.size,
+                                      _ptr(local), C.byref(nb), C.byref(ns), C.byref(ms)))
     return Tiles(desc[: 8 * nb.value].reshape(-1, 8), stage[: ns.value],
                  local[: csr.n_entries], ms.value)
 

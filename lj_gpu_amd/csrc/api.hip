@@ -36,6 +36,7 @@ void reset_list(ljgpu_ctx* ctx) {
   ctx->tiles_ordered = false;
   ctx->tiles.tile_particles = 0;
   ctx->tiles.blocks = 0;
+  ctx->tiles.sliced_lanes = ctx->tiles.sliced_unroll = 0;
 }
 
 void release_all(ljgpu_ctx* ctx) {
@@ -77,22 +78,32 @@ int ensure_ell(ljgpu_ctx* ctx) {
 // dev_tile_order, which needs positions: a list built on the device gets it
 // at once, a list installed from the caller at its first force evaluation
 // (index order until then). Systems without a periodic box stay in index
-// order.
-int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
+// order. Tiles derived with positions also take the cutoff classes of cl2
+// (cut_partition) when the rows are bank-class ordered; a tiling derived
+// without positions is derived again, once, when positions first come.
+int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box, double cl2) {
   const int32_t want = ctx->eval_half ? 0 : effective_tile(ctx);
   const bool spatial = ctx->tile_order == LJGPU_TILE_ORDER_SPATIAL && want > 0;
-  if (want == ctx->tiles_for && (!spatial || ctx->tiles_ordered || !q)) return LJGPU_OK;
+  if (want == ctx->tiles_for && (ctx->tiles_ordered || !q || want == 0)) return LJGPU_OK;
   ctx->tiles.tile_particles = 0;
   ctx->tiles.blocks = 0;
+  ctx->tiles.sliced_lanes = ctx->tiles.sliced_unroll = 0;
   bool ordered = false;
   if (spatial && q && ctx->eval.n > 0)
     TRY(dev_tile_order(ctx->stream, *q, ctx->eval.n, box, ctx->rep_off, ctx->rep_boxes, want,
                        ctx->work, &ordered));
   if (ordered) TRY(dev_length_sort(ctx->stream, ctx->eval, want, ctx->rep_off, ctx->work));
   const int32_t* member = ordered ? ctx->work.order.as<int32_t>() : nullptr;
+  CutHint cut;
+  if (q) {
+    cut.q = *q;
+    cut.box = box;
+    cut.rep_box = ctx->rep_box_dev.as<double>();
+    cut.cl2 = cl2;
+  }
   for (int32_t tp = want; tp >= 1 && ctx->eval.n > 0; tp /= 2) {
     TRY(dev_make_tiles(ctx->stream, ctx->eval, tp, ctx->rep_off, member, ctx->bank_order,
-                       ctx->work, ctx->tiles));
+                       q ? &cut : nullptr, ctx->work, ctx->tiles));
     if (ctx->tiles.tile_particles > 0) break;
   }
   // The slot-order copies that calls of kSlotMinSteps or more steps run over
@@ -104,13 +115,25 @@ int ensure_tiles(ljgpu_ctx* ctx, const PosIn* q, double box) {
     TRY(dev_reserve(ctx->slot_p, bytes, ctx->stream));
   }
   ctx->tiles_for = want;
-  ctx->tiles_ordered = spatial && q;
+  ctx->tiles_ordered = q != nullptr;
   return LJGPU_OK;
 }
 
-int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q, double box) {
+// The tiles' offsets sliced for the configured lanes and unroll (built once
+// per tiling and configuration).
+int ensure_sliced(ljgpu_ctx* ctx) {
+  DevTiles& t = ctx->tiles;
+  if (t.blocks <= 0 || (t.sliced_lanes == ctx->lanes && t.sliced_unroll == ctx->unroll))
+    return LJGPU_OK;
+  const int32_t* sptr = t.order == LJGPU_TILE_ORDER_SPATIAL ? t.slot_ptr.as<int32_t>()
+                                                            : ctx->eval.ptr.as<int32_t>();
+  return dev_slice_tiles(ctx->stream, sptr, ctx->lanes, ctx->unroll, ctx->work, t);
+}
+
+int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q, double box, double cl2) {
   if (ctx->lanes == 1) return ensure_ell(ctx);
-  return ensure_tiles(ctx, q, box);
+  TRY(ensure_tiles(ctx, q, box, cl2));
+  return ensure_sliced(ctx);
 }
 
 PosIn positions_of(const ForceArgs& fa) {
@@ -268,7 +291,7 @@ int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps, doubl
 // Kernel arguments of one force evaluation over the installed list.
 int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* lc) {
   const PosIn in = positions_of(fa);
-  TRY(prepare_layouts(ctx, &in, fa.box));
+  TRY(prepare_layouts(ctx, &in, fa.box, fa.cl2));
   *lv = ListView();
   lv->n = ctx->n;
   lv->nop = ctx->eval.nop.as<int32_t>();
@@ -297,6 +320,13 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
       lv->tile_desc = ctx->tiles.desc.as<int32_t>();
       lv->tile_stage = ctx->tiles.stage.as<int32_t>();
       lv->tile_local = ctx->tiles.local.as<uint16_t>();
+      if (ctx->tiles.sliced_lanes > 0) {
+        lv->tile_sliced = ctx->tiles.sliced.as<uint16_t>();
+        lv->tile_wave_off = ctx->tiles.wave_off.as<int32_t>();
+        lv->tile_waves_per_block = ctx->tiles.waves_per_block;
+        lv->tile_sliced_lanes = ctx->tiles.sliced_lanes;
+        lv->tile_sliced_unroll = ctx->tiles.sliced_unroll;
+      }
       if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL) {
         lv->tile_member = ctx->tiles.member.as<int32_t>();
         lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
@@ -383,6 +413,11 @@ StepKey step_key(const ForceArgs& fa, const ListView& lv, const LaunchConfig& lc
   k.add(lv.tile_member);
   k.add(lv.tile_slot_ptr);
   k.add(lv.tile_stage_slot);
+  k.add(lv.tile_sliced);
+  k.add(lv.tile_wave_off);
+  k.add(lv.tile_waves_per_block);
+  k.add(lv.tile_sliced_lanes);
+  k.add(lv.tile_sliced_unroll);
   k.add(lv.replica_box);
   k.add(lc.lanes);
   k.add(lc.unroll);

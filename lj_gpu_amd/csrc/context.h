@@ -105,7 +105,7 @@ struct ljgpu_ctx {
   int cus = 0;                    // compute units of the device
   int32_t tiles_for = 0;          // effective tile size the tiling was derived for (0: none)
   int tile_order = LJGPU_TILE_ORDER_SPATIAL;  // requested order of the particles in tiles
-  bool tiles_ordered = false;     // the tiling had positions to take that order from
+  bool tiles_ordered = false;     // the tiling was derived with positions (order, cutoff classes)
   bool bank_order = true;         // tile rows in the LDS bank-class order (bank_class_order)
   ljgpu::DevTiles tiles;
   ljgpu::DevMem slot_q, slot_p;   // q and p in slot order during a many-step call
@@ -165,8 +165,11 @@ void reset_list(ljgpu_ctx* ctx);
 // Particles per tile for the current kernel configuration (0: untiled).
 int32_t effective_tile(const ljgpu_ctx* ctx);
 // Derive the layouts the configured kernel needs. Positions (q, with the
-// periodic box, 0 for none) let the tiles take the spatial order.
-int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q = nullptr, double box = 0.0);
+// periodic box, 0 for none) let the tiles take the spatial order and the
+// cutoff classes of cl2 (the reference's squared cutoff, 9.0, by default).
+constexpr double kDefaultCutHint = 9.0;
+int prepare_layouts(ljgpu_ctx* ctx, const PosIn* q = nullptr, double box = 0.0,
+                    double cl2 = kDefaultCutHint);
 // Device positions of a force evaluation's arguments.
 PosIn positions_of(const ForceArgs& fa);
 // Build the list from device positions and install it (ljgpu_build_list).

@@ -229,6 +229,20 @@ __device__ __forceinline__ T lds_at(uint32_t a) {
   return *reinterpret_cast<const __attribute__((address_space(3))) T*>(static_cast<uintptr_t>(a));
 }
 
+// One entry of a tile row: in FP64 the unit force scalar (lj_df_unit_f64,
+// the row's sum is scaled by 24*dt in tile_body), otherwise pair_compute.
+template <int PREC, bool MI>
+__device__ __forceinline__ void tile_pair(double dx, double dy, double dz, const PairParams& pp,
+                                          Accum<PREC>& acc) {
+  if constexpr (PREC == kF64) {
+    min_image<MI>(dx, dy, dz, pp);
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    acc.add(lj_df_unit_f64(rcp_f64(r2), !(r2 > pp.cl2)), 0.0f, dx, dy, dz);
+  } else {
+    pair_compute<PREC, MI, false>(MomAos{nullptr}, 0, true, dx, dy, dz, pp, acc);
+  }
+}
+
 // One row of an LDS tile: entries [0, np) of the row, lane `sub` of the
 // segment taking entries base + u*TPI + sub. The row's entry offsets
 // (`lrow`) and the partner positions (`lb`) are both in LDS. Full batches of
@@ -251,7 +265,7 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
       const double xj = lds_at<double>(off[u]);
       const double yj = lds_at<double>(off[u] + 8);
       const double zj = lds_at<double>(off[u] + 16);
-      pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+      tile_pair<PREC, MI>(xj - xi, yj - yi, zj - zi, pp, acc);
     }
   }
   for (int k = base + sub; k < np; k += TPI) {
@@ -259,7 +273,7 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
     const double xj = lds_at<double>(o);
     const double yj = lds_at<double>(o + 8);
     const double zj = lds_at<double>(o + 16);
-    pair_compute<PREC, MI, false>(mom, 0, true, xj - xi, yj - yi, zj - zi, pp, acc);
+    tile_pair<PREC, MI>(xj - xi, yj - yi, zj - zi, pp, acc);
   }
 }
 
@@ -271,62 +285,108 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
 // they read beyond a row is never used. LIMIT masks entries at or beyond
 // `np` (the last, partial batch of a row): their offsets are replaced by 0
 // (a valid slot) and their force by 0.
-template <int TPI, int U, int PREC, bool MI, bool LIMIT, bool REFILL, class M>
+// One batch of a tile row: all U partner positions are read from LDS before
+// any is used, then the U entries are computed; the offsets buffer `o` is
+// refilled in between with the batch two ahead (loads from global memory
+// that then have two batches of arithmetic to land). The offset arrays are
+// padded so these loads stay inside them past the end of the last row; what
+// they read beyond a row is never used. LIMIT masks entries at or beyond
+// `np` (the last, partial batch of a row): their offsets are replaced by 0
+// (a valid slot) and their force by 0. `base` is the batch's first entry of
+// the row; `refill` the layout index of the batch two ahead, whose slot u
+// sits at refill + u * US + LS (LS: the lane's own offset within a slot).
+// A last batch in which no active lane holds an entry within the cutoff
+// skips the force arithmetic (the wavefront branches as a whole): the
+// reference's `if (r2 > CL2) continue;` for a whole batch. Rows hold their
+// entries in cutoff classes (cut_partition), so the entries beyond the
+// cutoff collect at the rows' ends. In FP64 the force scalar omits the
+// factor 24*dt (lj_df_unit_f64), which tile_body applies to the row's sum.
+template <int TPI, int U, int PREC, bool MI, bool LIMIT, bool REFILL, int US, class M>
 __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restrict__ lrow,
-                                           uint32_t (&o)[U], int base, int refill, int np,
-                                           int sub, double xi, double yi, double zi,
+                                           uint32_t (&o)[U], int base, int refill, int ls,
+                                           int np, int sub, double xi, double yi, double zi,
                                            const PairParams& pp, Accum<PREC>& acc) {
-  double xj[U], yj[U], zj[U];
+  double dx[U], dy[U], dz[U], r2[U];
   bool valid[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
     valid[u] = !LIMIT || base + u * TPI + sub < np;
     const uint32_t a = kTileLdsHeader + (valid[u] ? o[u] : 0u);
-    xj[u] = lds_at<double>(a);
-    yj[u] = lds_at<double>(a + 8);
-    zj[u] = lds_at<double>(a + 16);
+    dx[u] = lds_at<double>(a);
+    dy[u] = lds_at<double>(a + 8);
+    dz[u] = lds_at<double>(a + 16);
   }
   if constexpr (REFILL) {
 #pragma unroll
-    for (int u = 0; u < U; u++) o[u] = lrow[refill + u * TPI + sub];
+    for (int u = 0; u < U; u++) o[u] = lrow[refill + u * US + ls];
+  }
+  bool in[U];
+  bool inside = false;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    dx[u] -= xi;
+    dy[u] -= yi;
+    dz[u] -= zi;
+    min_image<MI>(dx[u], dy[u], dz[u], pp);
+    r2[u] = fma(dz[u], dz[u], fma(dy[u], dy[u], dx[u] * dx[u]));
+    in[u] = valid[u] && !(r2[u] > pp.cl2);
+    inside |= in[u];
+  }
+  if constexpr (LIMIT) {
+    if (!__any(inside)) return;
   }
 #pragma unroll
-  for (int u = 0; u < U; u++)
-    pair_compute<PREC, MI, false>(mom, 0, valid[u], xj[u] - xi, yj[u] - yi, zj[u] - zi, pp, acc);
+  for (int u = 0; u < U; u++) {
+    if constexpr (PREC == kF64) {
+      acc.add(lj_df_unit_f64(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]), 0.0f, dx[u],
+              dy[u], dz[u]);
+    } else {
+      float dff = lj_df_mixed(r2[u], pp);
+      if constexpr (LIMIT) dff = valid[u] ? dff : 0.0f;
+      acc.add(static_cast<double>(dff), dff, dx[u], dy[u], dz[u]);
+    }
+  }
 }
 
-// As tile_row, with the row's entry offsets read from global memory
-// (`lrow` in HBM) through two alternating offset buffers, each refilled two
-// batches ahead, and the row's last partial batch run as one masked batch.
-template <int TPI, int U, int PREC, bool MI, class M>
+// As tile_row, with the row's entry offsets read from global memory through
+// two alternating offset buffers, each refilled two batches ahead, and the
+// row's last partial batch run as one masked batch. SL selects the offset
+// layout: false, the row's own offsets (`lrow` at the row's first entry,
+// entry k at k); true, the wavefront slices of dev_slice_tiles (`lrow` at
+// the lane's place in its wavefront's slice, batch j slot u at
+// j * 64 * U + u * 64), so each offset load reads 64 consecutive values.
+template <int TPI, int U, int PREC, bool MI, bool SL, class M>
 __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __restrict__ lrow,
                                                 int np, int sub, double xi, double yi, double zi,
                                                 const PairParams& pp, Accum<PREC>& acc) {
   constexpr int kB = U * TPI;
+  constexpr int JS = SL ? 64 * U : kB;  // layout stride of a batch
+  constexpr int US = SL ? 64 : TPI;     // layout stride of a slot
+  const int ls = SL ? 0 : sub;
   const int nfull = np / kB;
   uint32_t oa[U], ob[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    oa[u] = lrow[u * TPI + sub];
-    ob[u] = lrow[kB + u * TPI + sub];
+    oa[u] = lrow[u * US + ls];
+    ob[u] = lrow[JS + u * US + ls];
   }
   int b = 0;
   for (; b + 2 <= nfull; b += 2) {
-    tile_batch<TPI, U, PREC, MI, false, true>(mom, lrow, oa, b * kB, (b + 2) * kB, np, sub, xi,
-                                              yi, zi, pp, acc);
-    tile_batch<TPI, U, PREC, MI, false, true>(mom, lrow, ob, (b + 1) * kB, (b + 3) * kB, np, sub,
-                                              xi, yi, zi, pp, acc);
+    tile_batch<TPI, U, PREC, MI, false, true, US>(mom, lrow, oa, b * kB, (b + 2) * JS, ls, np,
+                                                  sub, xi, yi, zi, pp, acc);
+    tile_batch<TPI, U, PREC, MI, false, true, US>(mom, lrow, ob, (b + 1) * kB, (b + 3) * JS, ls,
+                                                  np, sub, xi, yi, zi, pp, acc);
   }
   if (b < nfull) {
-    tile_batch<TPI, U, PREC, MI, false, false>(mom, lrow, oa, b * kB, 0, np, sub, xi, yi, zi, pp,
-                                               acc);
+    tile_batch<TPI, U, PREC, MI, false, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi, yi,
+                                                   zi, pp, acc);
     b++;
 #pragma unroll
     for (int u = 0; u < U; u++) oa[u] = ob[u];
   }
   if (b * kB < np)
-    tile_batch<TPI, U, PREC, MI, true, false>(mom, lrow, oa, b * kB, 0, np, sub, xi, yi, zi, pp,
-                                              acc);
+    tile_batch<TPI, U, PREC, MI, true, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi, yi,
+                                                  zi, pp, acc);
 }
 
 // LDS-tiled full-list kernel: one workgroup per block of consecutive slots
@@ -353,13 +413,15 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __
 // smaller (persistent) grid each loops over several, so resident workgroups
 // drift out of phase and one's staging and memory stalls overlap another's
 // arithmetic.
-template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS, bool SL>
 __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restrict__ nop,
                                           const int32_t* __restrict__ member,
                                           const int32_t* __restrict__ slot_ptr,
                                           const int4* __restrict__ desc,
                                           const int32_t* __restrict__ stage,
-                                          const uint16_t* __restrict__ local, int b,
+                                          const uint16_t* __restrict__ local,
+                                          const uint16_t* __restrict__ sliced,
+                                          const int32_t* __restrict__ wave_off, int wpb, int b,
                                           const PairParams& gpp, double greach,
                                           const double2* __restrict__ rep_box, char* lb,
                                           int* flag) {
@@ -459,28 +521,38 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
       else
         tile_row<TPI, U, PREC, false>(mom, loff + rs, np, sub, xi, yi, zi, pp, acc);
     } else {
-      const uint16_t* grow = local + eoff + rs;
+      const uint16_t* grow =
+          SL ? sliced + wave_off[static_cast<int64_t>(b) * wpb + threadIdx.x / 64] +
+                   (threadIdx.x & 63)
+             : local + eoff + rs;
       if (PBC && any_far)
-        tile_row_stream<TPI, U, PREC, true>(mom, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, true, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc);
       else
-        tile_row_stream<TPI, U, PREC, false>(mom, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc);
     }
   }
-  const double fx = segment_sum<TPI>(acc.vx());
-  const double fy = segment_sum<TPI>(acc.vy());
-  const double fz = segment_sum<TPI>(acc.vz());
+  double fx = segment_sum<TPI>(acc.vx());
+  double fy = segment_sum<TPI>(acc.vy());
+  double fz = segment_sum<TPI>(acc.vz());
+  if constexpr (PREC == kF64) {
+    fx *= pp.c24;
+    fy *= pp.c24;
+    fz *= pp.c24;
+  }
   if (active && sub == 0) mom.add(i, fx, fy, fz);
 }
 
-// At most 80 VGPRs (six waves per SIMD); the unconstrained allocation of 84
-// gives five, measured 2-3 % slower.
-template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+// At most 96 VGPRs (five waves per SIMD): holding every entry's distance of a
+// batch until the batch's cutoff test needs more than the 80 of six waves,
+// which spilled and measured 5 % slower.
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS, bool SL>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5)))
 k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ member,
        const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
        const int32_t* __restrict__ stage,
-       const uint16_t* __restrict__ local, int32_t first, int32_t nblocks, PairParams pp,
-       double reach, const double2* __restrict__ rep_box) {
+       const uint16_t* __restrict__ local, const uint16_t* __restrict__ sliced,
+       const int32_t* __restrict__ wave_off, int32_t wpb, int32_t first, int32_t nblocks,
+       PairParams pp, double reach, const double2* __restrict__ rep_box) {
   // Dynamic LDS only (no static LDS, so its base is address 0 and constant
   // offsets fold into the LDS instructions): a 16-byte header holding the
   // frame-check word, then the tile's positions (and offsets).
@@ -494,8 +566,9 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   const int stride = static_cast<int>(gridDim.x) / kXcds;
   const int t_end = min(nblocks, (xcd + 1) * per);
   for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
-    tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS>(pos, mom, nop, member, slot_ptr, desc, stage,
-                                                 local, first + b, pp, reach, rep_box, lb, flag);
+    tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS, SL>(pos, mom, nop, member, slot_ptr, desc, stage,
+                                                     local, sliced, wave_off, wpb, first + b, pp,
+                                                     reach, rep_box, lb, flag);
     __syncthreads();  // the next tile overwrites this tile's LDS
   }
 }
@@ -531,7 +604,7 @@ int run_ell(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& p
   return launch_status("ELL force kernel launch");
 }
 
-template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS>
+template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS, bool SL>
 int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
              const LaunchConfig& lc) {
   const int threads = (lv.tile_particles * TPI + 63) / 64 * 64;
@@ -547,25 +620,31 @@ int run_tile(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& 
       kTileLdsHeader + (OFFS_LDS ? static_cast<size_t>(lv.tile_lds_bytes)
                                  : static_cast<size_t>(lv.tile_max_stage) * 24);
   const double reach = 0.5 * pp.box - sqrt(pp.cl2);
-  hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS>), dim3(grid), dim3(threads),
+  hipLaunchKernelGGL((k_tile<TPI, U, PREC, P, M, PBC, OFFS_LDS, SL>), dim3(grid), dim3(threads),
                      lds_bytes, s, pos, mom, lv.nop, lv.tile_member, lv.tile_slot_ptr,
                      reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
-                     lv.tile_first, lv.tile_blocks, pp, reach,
-                     reinterpret_cast<const double2*>(lv.replica_box));
+                     lv.tile_sliced, lv.tile_wave_off, lv.tile_waves_per_block, lv.tile_first,
+                     lv.tile_blocks, pp, reach, reinterpret_cast<const double2*>(lv.replica_box));
   return launch_status("tiled force kernel launch");
+}
+
+template <int TPI, int U, int PREC, class P, class M, bool PBC>
+int run_tile_layout(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
+                    const LaunchConfig& lc) {
+  if (lc.tile_offsets_lds) return run_tile<TPI, U, PREC, P, M, PBC, true, false>(s, pos, mom, lv, pp, lc);
+  // The wavefront slices serve the launch they were cut for.
+  const bool sl = lv.tile_sliced && lv.tile_sliced_lanes == TPI && lv.tile_sliced_unroll == U;
+  return sl ? run_tile<TPI, U, PREC, P, M, PBC, false, true>(s, pos, mom, lv, pp, lc)
+            : run_tile<TPI, U, PREC, P, M, PBC, false, false>(s, pos, mom, lv, pp, lc);
 }
 
 template <int TPI, int PREC, class P, class M, bool PBC>
 int run_tile_unroll(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams& pp,
                     const LaunchConfig& lc) {
-  const bool o = lc.tile_offsets_lds;
   switch (lc.unroll) {
-    case 1: return o ? run_tile<TPI, 1, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
-                     : run_tile<TPI, 1, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
-    case 2: return o ? run_tile<TPI, 2, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
-                     : run_tile<TPI, 2, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
-    case 4: return o ? run_tile<TPI, 4, PREC, P, M, PBC, true>(s, pos, mom, lv, pp, lc)
-                     : run_tile<TPI, 4, PREC, P, M, PBC, false>(s, pos, mom, lv, pp, lc);
+    case 1: return run_tile_layout<TPI, 1, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 2: return run_tile_layout<TPI, 2, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
+    case 4: return run_tile_layout<TPI, 4, PREC, P, M, PBC>(s, pos, mom, lv, pp, lc);
     default: return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unroll must be 1, 2 or 4");
   }
 }

@@ -117,7 +117,7 @@ void csr_make_row_ordered(HostCSR* c) {
 }
 
 bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* member,
-                 HostTiles* out) {
+                 const double* q, double box, double cl2, HostTiles* out) {
   const int32_t n = full.n;
   out->tile_particles = tile_particles;
   out->max_stage = 0;
@@ -174,7 +174,22 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* mem
         out->local[sptr[s] + k] = static_cast<uint16_t>(rank * 24);
       }
       uint16_t buf[kBankRowMax], cnt[kBankClasses], nxt[kBankClasses];
-      bank_class_order(out->local.data() + sptr[s], full.nop[i], s - i0, buf, cnt, nxt);
+      uint16_t* row = out->local.data() + sptr[s];
+      const int m = full.nop[i];
+      int ni = 0;
+      if (q && m <= kBankRowMax) {
+        bool inner[kBankRowMax];
+        for (int k = 0; k < m; k++) {
+          const int32_t j = full.list[kp + k];
+          inner[k] = cut_inner(q[4 * static_cast<int64_t>(i)], q[4 * static_cast<int64_t>(i) + 1],
+                               q[4 * static_cast<int64_t>(i) + 2], q[4 * static_cast<int64_t>(j)],
+                               q[4 * static_cast<int64_t>(j) + 1],
+                               q[4 * static_cast<int64_t>(j) + 2], box, cl2);
+        }
+        ni = cut_partition(row, m, inner, buf);
+      }
+      bank_class_order(row, ni, s - i0, 0, buf, cnt, nxt);
+      bank_class_order(row + ni, m - ni, s - i0, ni, buf, cnt, nxt);
     }
     i0 += ni;
   }
@@ -456,13 +471,16 @@ int ljgpu_random_shfl(int32_t n, const int32_t* nop, const int32_t* ptr, int32_t
   return LJGPU_OK;
 }
 
-int ljgpu_make_tiles_ordered(int32_t n, const int32_t* nop, const int32_t* ptr,
-                             const int32_t* list, int64_t n_entries, int32_t tile_particles,
-                             const int32_t* member, int32_t* desc, int64_t desc_capacity,
-                             int32_t* stage, int64_t stage_capacity, uint16_t* local,
-                             int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
+int ljgpu_make_tiles_cut(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,
+                         int64_t n_entries, int32_t tile_particles, const int32_t* member,
+                         const double* q, double box, double cl2, int32_t* desc,
+                         int64_t desc_capacity, int32_t* stage, int64_t stage_capacity,
+                         uint16_t* local, int32_t* n_blocks, int64_t* n_stage,
+                         int32_t* max_stage) {
   if (!n_blocks || !n_stage || !max_stage || tile_particles <= 0 || tile_particles > 1024)
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_tiles: bad argument");
+  if (q && !(cl2 >= 0.0))
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "make_tiles: cl2 must be >= 0");
   int rc = validate_csr(n, nop, ptr, list, n_entries, LJGPU_LIST_FULL);
   if (rc) return rc;
   if (member) {
@@ -481,7 +499,7 @@ int ljgpu_make_tiles_ordered(int32_t n, const int32_t* nop, const int32_t* ptr,
   if (!csr_is_row_ordered(c))
     return set_error(LJGPU_ERR_BAD_LIST, "make_tiles: rows must be stored in order");
   HostTiles t;
-  if (!build_tiles(c, tile_particles, member, &t))
+  if (!build_tiles(c, tile_particles, member, q, box, cl2, &t))
     return set_error(LJGPU_ERR_CAPACITY, "make_tiles: a row does not fit one tile");
   *n_blocks = static_cast<int32_t>(t.desc.size() / 8);
   *n_stage = static_cast<int64_t>(t.stage.size());
@@ -494,6 +512,16 @@ int ljgpu_make_tiles_ordered(int32_t n, const int32_t* nop, const int32_t* ptr,
   std::memcpy(stage, t.stage.data(), t.stage.size() * sizeof(int32_t));
   std::memcpy(local, t.local.data(), t.local.size() * sizeof(uint16_t));
   return LJGPU_OK;
+}
+
+int ljgpu_make_tiles_ordered(int32_t n, const int32_t* nop, const int32_t* ptr,
+                             const int32_t* list, int64_t n_entries, int32_t tile_particles,
+                             const int32_t* member, int32_t* desc, int64_t desc_capacity,
+                             int32_t* stage, int64_t stage_capacity, uint16_t* local,
+                             int32_t* n_blocks, int64_t* n_stage, int32_t* max_stage) {
+  return ljgpu_make_tiles_cut(n, nop, ptr, list, n_entries, tile_particles, member, nullptr, 0.0,
+                              0.0, desc, desc_capacity, stage, stage_capacity, local, n_blocks,
+                              n_stage, max_stage);
 }
 
 int ljgpu_make_tiles(int32_t n, const int32_t* nop, const int32_t* ptr, const int32_t* list,

@@ -58,6 +58,13 @@ struct DevTiles {
   DevMem stage_slot;  // stage as slots (spatial order only)
   // Each block's staged particles are sorted by slot (by index in index
   // order), and local holds 24 * the rank of each entry's partner among them.
+
+  // The same offsets sliced per wavefront for the streamed tile kernel of
+  // sliced_lanes lanes per particle and sliced_unroll entries per lane per
+  // batch (see dev_slice_tiles); sliced_lanes = 0: not built.
+  DevMem sliced;      // uint16
+  DevMem wave_off;    // int32 per (block, wavefront slot): start of its slice
+  int32_t sliced_lanes = 0, sliced_unroll = 0, waves_per_block = 0;
 };
 void dev_release(DevTiles& t);
 
@@ -144,9 +151,31 @@ int dev_make_ell(hipStream_t s, const DevCSR& c, int32_t width, DevMem& ell);
 // blocks cover; null takes them in index order. bank_order puts each row's
 // entries in the LDS bank-class order (bank_class_order), as the host model
 // does; without it rows keep the list order.
+// With bank_order and positions in `cut` (cut->q.x non-null), the rows
+// first take the cutoff classes (cut_partition) of those positions.
+struct CutHint {
+  PosIn q;                        // positions in the caller's particle order
+  double box = 0.0;               // single system: periodic box (<= 0 open)
+  const double* rep_box = nullptr;  // replicas: (box, 1/box) per replica on the device
+  double cl2 = 9.0;
+};
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t tile_particles,
                    const std::vector<int32_t>& replicas, const int32_t* member, bool bank_order,
-                   ListWork& w, DevTiles& out);
+                   const CutHint* cut, ListWork& w, DevTiles& out);
+
+// The tiles' entry offsets sliced per wavefront, for a tile kernel in
+// which `lanes` consecutive lanes own one row and take entries in batches
+// of lanes * unroll (entry k of a row: batch k / (lanes * unroll), slot
+// (k / lanes) % unroll, lane k % lanes). Wavefront w of block b holds the
+// block's rows [w * 64/lanes, (w+1) * 64/lanes) and owns the slice that
+// starts at wave_off[b * waves_per_block + w]: for each of its batches
+// (as many as its longest row needs) 64 * unroll offsets, slot by slot,
+// each slot the 64 lanes' entries in lane order, so one offset load of the
+// kernel reads 64 consecutive values. Slots past a row's end hold 0.
+// Needs local, desc and the slot row starts of a built tiling; a slicing
+// whose total does not fit int32 is not built (sliced_lanes stays 0).
+int dev_slice_tiles(hipStream_t s, const int32_t* slot_ptr, int32_t lanes, int32_t unroll,
+                    ListWork& w, DevTiles& t);
 
 // Spatial particle order for LDS tiles, into w.order: each periodic system
 // (the whole set, or every replica of `replicas` with its box) is cut into

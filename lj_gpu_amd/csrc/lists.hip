@@ -1107,22 +1107,93 @@ __global__ __launch_bounds__(kBlock) void k_tile_fill(const int32_t* __restrict_
   }
 }
 
-// The bank-class order of every tile row (bank_class_order): one wavefront
-// per tile, one row per lane, each lane's buffers in LDS.
+// The row order of every tile row: with positions (cut.q.x non-null) the
+// cutoff classes (cut_partition over cut_inner, partners found through the
+// block's staged particles), then the bank-class order of each class
+// (bank_class_order). One wavefront per tile, one row per lane, each lane's
+// buffers in LDS.
+struct CutDev {
+  PosIn q;
+  const int32_t* desc;
+  const int32_t* stage;
+  const int32_t* member;
+  double box;
+  const double* rep_box;
+  double cl2;
+};
 constexpr int kBankBlock = 64;
 __global__ __launch_bounds__(kBankBlock) void k_bank_order(const int32_t* __restrict__ geo,
                                                            int32_t nblocks,
                                                            const int32_t* __restrict__ slot_ptr,
-                                                           uint16_t* __restrict__ local) {
+                                                           uint16_t* __restrict__ local,
+                                                           CutDev cut) {
   __shared__ uint16_t buf[kBankBlock][kBankRowMax];
   __shared__ uint16_t cnt[kBankBlock][kBankClasses];
   __shared__ uint16_t nxt[kBankBlock][kBankClasses];
   const int t = static_cast<int>(threadIdx.x);
+  const int64_t st = cut.q.stride;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const int32_t i0 = geo[4 * b], ni = geo[4 * b + 1];
     for (int32_t r = t; r < ni; r += kBankBlock) {
       const int32_t a = slot_ptr[i0 + r];
-      bank_class_order(local + a, slot_ptr[i0 + r + 1] - a, r, buf[t], cnt[t], nxt[t]);
+      const int m = slot_ptr[i0 + r + 1] - a;
+      uint16_t* row = local + a;
+      int nin = 0;
+      if (cut.q.x && m <= kBankRowMax) {
+        const int64_t i = cut.member ? cut.member[i0 + r] : i0 + r;
+        const double xi = cut.q.x[i * st], yi = cut.q.y[i * st], zi = cut.q.z[i * st];
+        const int32_t soff = cut.desc[8 * static_cast<int64_t>(b) + 2];
+        const double box = cut.rep_box ? cut.rep_box[2 * geo[4 * b + 2]] : cut.box;
+        int nout = 0;
+        for (int k = 0; k < m; k++) {
+          const uint16_t o = row[k];
+          const int64_t j = cut.stage[soff + o / 24];
+          if (cut_inner(xi, yi, zi, cut.q.x[j * st], cut.q.y[j * st], cut.q.z[j * st], box,
+                        cut.cl2))
+            row[nin++] = o;
+          else
+            buf[t][nout++] = o;
+        }
+        for (int k = 0; k < nout; k++) row[nin + k] = buf[t][k];
+      }
+      bank_class_order(row, nin, r, 0, buf[t], cnt[t], nxt[t]);
+      bank_class_order(row + nin, m - nin, r, nin, buf[t], cnt[t], nxt[t]);
+    }
+  }
+}
+
+// Slice lengths: per (block, wavefront slot), 64 * unroll offsets for each
+// batch its longest row needs (0 for slots past the block's rows).
+__global__ void k_wave_slices(const int32_t* __restrict__ desc, int32_t nblocks, int32_t wpb,
+                              const int32_t* __restrict__ slot_ptr, int32_t lanes, int32_t unroll,
+                              int32_t* __restrict__ len) {
+  const int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (id >= static_cast<int64_t>(nblocks) * wpb) return;
+  const int32_t b = static_cast<int32_t>(id / wpb), w = static_cast<int32_t>(id % wpb);
+  const int32_t i0 = desc[8 * static_cast<int64_t>(b)], ni = desc[8 * static_cast<int64_t>(b) + 1];
+  const int32_t rpw = 64 / lanes, kb = lanes * unroll;
+  int32_t batches = 0;
+  for (int32_t r = w * rpw; r < min(ni, (w + 1) * rpw); r++) {
+    const int32_t np = slot_ptr[i0 + r + 1] - slot_ptr[i0 + r];
+    batches = max(batches, (np + kb - 1) / kb);
+  }
+  len[id] = batches * 64 * unroll;
+}
+
+// Every entry offset to its slice position (one workgroup per block).
+__global__ __launch_bounds__(kBlock) void k_slice_scatter(
+    const int32_t* __restrict__ desc, int32_t nblocks, int32_t wpb,
+    const int32_t* __restrict__ slot_ptr, const uint16_t* __restrict__ local, int32_t lanes,
+    int32_t unroll, const int32_t* __restrict__ wave_off, uint16_t* __restrict__ sliced) {
+  const int32_t rpw = 64 / lanes, kb = lanes * unroll;
+  for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const int32_t i0 = desc[8 * static_cast<int64_t>(b)], ni = desc[8 * static_cast<int64_t>(b) + 1];
+    for (int32_t r = 0; r < ni; r++) {
+      const int32_t a = slot_ptr[i0 + r], np = slot_ptr[i0 + r + 1] - a;
+      const int64_t base = wave_off[static_cast<int64_t>(b) * wpb + r / rpw] + (r % rpw) * lanes;
+      for (int32_t k = threadIdx.x; k < np; k += kBlock)
+        sliced[base + static_cast<int64_t>(k / kb) * 64 * unroll + ((k / lanes) % unroll) * 64 +
+               k % lanes] = local[a + k];
     }
   }
 }
@@ -1297,6 +1368,9 @@ void dev_release(DevTiles& t) {
   dev_release(t.desc);
   dev_release(t.stage);
   dev_release(t.local);
+  dev_release(t.sliced);
+  dev_release(t.wave_off);
+  t.sliced_lanes = t.sliced_unroll = t.waves_per_block = 0;
   dev_release(t.member);
   dev_release(t.slot_ptr);
   dev_release(t.stage_slot);
@@ -1725,7 +1799,8 @@ int dev_length_sort(hipStream_t s, const DevCSR& c, int32_t T, const std::vector
 }
 
 int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<int32_t>& reps,
-                   const int32_t* member, bool bank_order, ListWork& w, DevTiles& out) {
+                   const int32_t* member, bool bank_order, const CutHint* cut, ListWork& w,
+                   DevTiles& out) {
   out.tile_particles = 0;
   out.blocks = 0;
   if (T <= 0 || T > kTileRowsMax || c.n <= 0) return LJGPU_OK;
@@ -1829,7 +1904,17 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
     out.stage_stride = 0;
   }
   if (bank_order) {
-    k_bank_order<<<grid, kBankBlock, 0, s>>>(dgeo, nb, sptr, out.local.as<uint16_t>());
+    CutDev cd{};
+    if (cut && cut->q.x) {
+      cd.q = cut->q;
+      cd.desc = out.desc.as<int32_t>();
+      cd.stage = out.stage.as<int32_t>();
+      cd.member = mem;
+      cd.box = cut->box;
+      cd.rep_box = reps.empty() ? nullptr : cut->rep_box;
+      cd.cl2 = cut->cl2;
+    }
+    k_bank_order<<<grid, kBankBlock, 0, s>>>(dgeo, nb, sptr, out.local.as<uint16_t>(), cd);
     LTRY(hipGetLastError(), "tile bank order kernel");
   }
   out.tile_particles = T;
@@ -1837,6 +1922,35 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
   out.staged = staged;
   out.max_stage = h[1];
   out.lds_bytes = h[2];
+  return LJGPU_OK;
+}
+
+int dev_slice_tiles(hipStream_t s, const int32_t* slot_ptr, int32_t lanes, int32_t unroll,
+                    ListWork& w, DevTiles& t) {
+  t.sliced_lanes = t.sliced_unroll = t.waves_per_block = 0;
+  if (t.blocks <= 0 || lanes < 1 || lanes > 64 || (64 % lanes) != 0 || unroll < 1) return LJGPU_OK;
+  const int32_t wpb = (t.tile_particles * lanes + 63) / 64;
+  const int64_t nw = static_cast<int64_t>(t.blocks) * wpb;
+  LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(nw) + 1), s));
+  LCHECK(dev_reserve(t.wave_off, sizeof(int32_t) * (static_cast<size_t>(nw) + 1), s));
+  k_wave_slices<<<grid_for(nw, kBlock), kBlock, 0, s>>>(t.desc.as<int32_t>(), t.blocks, wpb,
+                                                        slot_ptr, lanes, unroll,
+                                                        w.counts.as<int32_t>());
+  LTRY(hipGetLastError(), "slice length kernel");
+  int64_t total = 0;
+  LCHECK(dev_scan(s, w.counts.as<int32_t>(), t.wave_off.as<int32_t>(), nw, w, &total));
+  // The kernel's offset prefetch reads up to two batches past a slice.
+  const int64_t pad = 2 * 64 * static_cast<int64_t>(unroll) + 64;
+  if (total + pad > INT32_MAX) return LJGPU_OK;
+  LCHECK(dev_reserve(t.sliced, sizeof(uint16_t) * (total + pad), s));
+  LCHECK(fill<uint16_t>(s, t.sliced.as<uint16_t>(), total + pad, 0));
+  k_slice_scatter<<<std::min(t.blocks, 65535), kBlock, 0, s>>>(
+      t.desc.as<int32_t>(), t.blocks, wpb, slot_ptr, t.local.as<uint16_t>(), lanes, unroll,
+      t.wave_off.as<int32_t>(), t.sliced.as<uint16_t>());
+  LTRY(hipGetLastError(), "slice scatter kernel");
+  t.sliced_lanes = lanes;
+  t.sliced_unroll = unroll;
+  t.waves_per_block = wpb;
   return LJGPU_OK;
 }
 

@@ -122,12 +122,26 @@ __device__ __forceinline__ double rcp_f64(double x) {
 // Beyond the cutoff only the high word of s is cleared: s becomes a
 // denormal of at most 2^-1042 whose third and fourth powers round to exactly
 // 0, so df is exactly 0 at the cost of one 32-bit select.
-__device__ __forceinline__ double lj_df_f64(double r2, const PairParams& pp) {
-  double s = rcp_f64(r2);
+__device__ __forceinline__ double lj_df_f64_s(double s, double r2, const PairParams& pp) {
   const int hi = (r2 > pp.cl2) ? 0 : __double2hiint(s);
   s = __hiloint2double(hi, __double2loint(s));
   const double s3 = s * s * s;
   return (s3 * s) * fma(-pp.c48, s3, pp.c24);
+}
+
+__device__ __forceinline__ double lj_df_f64(double r2, const PairParams& pp) {
+  return lj_df_f64_s(rcp_f64(r2), r2, pp);
+}
+
+// FP64 force scalar without the factor 24*dt: s^4 (1 - 2 s^3) with s = 1/r2,
+// exactly 0 where `inside` is false (only the high word of s is cleared, as
+// in lj_df_f64). The tiled kernels multiply each row's sum by 24*dt once, so
+// the constants here are instruction operands instead of registers.
+__device__ __forceinline__ double lj_df_unit_f64(double s, bool inside) {
+  const int hi = inside ? __double2hiint(s) : 0;
+  s = __hiloint2double(hi, __double2loint(s));
+  const double s3 = s * s * s;
+  return (s3 * s) * fma(-2.0, s3, 1.0);
 }
 
 // Mixed-precision force scalar: r2 (computed in FP64) rounded once to FP32,

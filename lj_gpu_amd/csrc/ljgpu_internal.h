@@ -66,6 +66,11 @@ struct ListView {
                                           // momenta are given in slot order
   const int32_t* tile_slot_ptr = nullptr; // row start of each slot (+1)
   const int32_t* tile_stage_slot = nullptr;  // stage as slots (spatial order; else null)
+  // The offsets sliced per wavefront (dev_slice_tiles) for tile_sliced_lanes
+  // lanes per particle and tile_sliced_unroll entries per batch; null: none.
+  const uint16_t* tile_sliced = nullptr;
+  const int32_t* tile_wave_off = nullptr;
+  int32_t tile_waves_per_block = 0, tile_sliced_lanes = 0, tile_sliced_unroll = 0;
 
   // Batched replicas (tiled kernel only): per replica (box, 1/box), indexed
   // by descriptor slot 6; null for a single system.
@@ -109,11 +114,13 @@ LJ_HD inline int64_t tile_lds_bytes(int64_t ns, int64_t ne) {
 // Entries of one class keep their list order. Rows longer than kBankRowMax
 // keep the list order. Kernels with other lane counts read the same rows
 // with a weaker spread; the result is the same set of pairs either way.
-// buf holds kBankRowMax entries, cnt and nxt kBankClasses counts each.
+// The row may be the part of a tile row that starts at entry k0 of it (the
+// classes then continue from there). buf holds kBankRowMax entries, cnt and
+// nxt kBankClasses counts each.
 constexpr int kBankClasses = 32;
 constexpr int kBankRowMax = 256;
-LJ_HD inline void bank_class_order(uint16_t* row, int m, int r, uint16_t* buf, uint16_t* cnt,
-                                   uint16_t* nxt) {
+LJ_HD inline void bank_class_order(uint16_t* row, int m, int r, int k0, uint16_t* buf,
+                                   uint16_t* cnt, uint16_t* nxt) {
   if (m < 2 || m > kBankRowMax) return;
   for (int c = 0; c < kBankClasses; c++) cnt[c] = 0;
   for (int k = 0; k < m; k++) cnt[(row[k] / 24) & (kBankClasses - 1)]++;
@@ -128,11 +135,53 @@ LJ_HD inline void bank_class_order(uint16_t* row, int m, int r, uint16_t* buf, u
   }
   // nxt[c] is now the end of class c in buf; its next entry is nxt[c] - cnt[c].
   for (int k = 0; k < m; k++) {
-    int c = (4 * (r & 7) + k) & (kBankClasses - 1);
+    int c = (4 * (r & 7) + k0 + k) & (kBankClasses - 1);
     while (cnt[c] == 0) c = (c + 1) & (kBankClasses - 1);
     row[k] = buf[nxt[c] - cnt[c]];
     cnt[c]--;
   }
+}
+
+// Cutoff classes of a tile row. When the tiles are built from positions,
+// each row holds first the entries whose pair lay within the cutoff
+// (r2 <= cl2) at that time, then the others, each part in list order and
+// then in the bank-class order (its classes continuing from the part's
+// start). A force batch whose entries are all beyond the cutoff in every
+// lane of a wavefront then skips the force arithmetic, which is the
+// reference's `if (r2 > CL2) continue;` taken by whole batches; every entry's
+// r2 is still evaluated at every step, so pairs that have moved inside the
+// cutoff since are computed wherever they sit. r2 is the reference's
+// unfused (dx*dx + dy*dy) + dz*dz over the threshold minimum image (box > 0)
+// of d = q_j - q_i, the same double on the host and the device. Rows longer
+// than kBankRowMax keep the list order.
+LJ_HD inline bool cut_inner(double xi, double yi, double zi, double xj, double yj, double zj,
+                            double box, double cl2) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  double d[3] = {xj - xi, yj - yi, zj - zi};
+  if (box > 0.0) {
+    const double h = 0.5 * box;
+    for (int c = 0; c < 3; c++) {
+      if (d[c] > h) d[c] -= box;
+      else if (d[c] < -h) d[c] += box;
+    }
+  }
+  const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+  return r2 <= cl2;
+}
+
+// Reorder a row as inner entries (inner[k] true), then the others, both in
+// their order; returns the inner count. buf holds kBankRowMax entries.
+LJ_HD inline int cut_partition(uint16_t* row, int m, const bool* inner, uint16_t* buf) {
+  int ni = 0, no = 0;
+  for (int k = 0; k < m; k++) {
+    const uint16_t o = row[k];
+    if (inner[k]) row[ni++] = o;
+    else buf[no++] = o;
+  }
+  for (int k = 0; k < no; k++) row[ni + k] = buf[k];
+  return ni;
 }
 
 // Cell grid of the Verlet-list builders: cells at least the search length
@@ -158,10 +207,12 @@ void csr_make_row_ordered(HostCSR* c);
 // Split the particles of a row-ordered list, taken in the order `member`
 // (one particle per slot; null: index order), into blocks of at most
 // `tile_particles` consecutive slots that fit the LDS budget (blocks shrink
-// where needed). local is indexed by the rows stored in slot order. Returns
+// where needed). local is indexed by the rows stored in slot order. With
+// positions q (AoS double[N][4]; null: none) the rows take the cutoff
+// classes (cut_inner with box, cl2) before the bank-class order. Returns
 // false if a single row does not fit.
 bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* member,
-                 HostTiles* out);
+                 const double* q, double box, double cl2, HostTiles* out);
 
 struct ForceArgs {
   bool soa = false;

@@ -124,7 +124,7 @@ int ljgpu_autotune(ljgpu_ctx* ctx, const double* q, double box, double cl2, int 
   ctx->tile_particles = best->tile_particles;
   ctx->tile_resident = best->resident_per_cu;
   const PosIn in = positions_of(fa);
-  TRY(prepare_layouts(ctx, &in, box));
+  TRY(prepare_layouts(ctx, &in, box, cl2));
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return LJGPU_OK;
 }

@@ -117,23 +117,25 @@ def column_order(q: np.ndarray, box: float, tile_particles: int = 64, offsets=No
 
 
 def bank_class_order(offs: np.ndarray, starts: np.ndarray, lengths: np.ndarray,
-                     pos: np.ndarray) -> np.ndarray:
+                     pos: np.ndarray, k0=None) -> np.ndarray:
     """Numpy restatement of the library's entry order within tile rows.
 
     offs: LDS byte offsets (24 * staged rank) in list order; row i spans
     offs[starts[i] : starts[i] + lengths[i]] and sits at position pos[i] of
     its tile. An entry's class is rank mod 32. Entry k of a row at position r
-    takes the first entry (in list order) of class (4 * (r & 7) + k) mod 32
-    that the row still holds, or of the next class up, cyclically, that it
-    does. Rows shorter than 2 or longer than 256 entries keep the list order.
-    All rows advance together, one entry index per step."""
+    takes the first entry (in list order) of class (4 * (r & 7) + k0 + k)
+    mod 32 that the row still holds, or of the next class up, cyclically,
+    that it does (k0: where the row starts within its tile row, 0 by
+    default). Rows shorter than 2 or longer than 256 entries keep the list
+    order. All rows advance together, one entry index per step."""
     offs = np.asarray(offs)
     out = offs.copy()
     starts, lengths, pos = (np.asarray(a, np.int64) for a in (starts, lengths, pos))
+    k0 = np.zeros_like(pos) if k0 is None else np.asarray(k0, np.int64)
     sel = np.nonzero((lengths >= 2) & (lengths <= 256))[0]
     if sel.size == 0:
         return out
-    st, ln, ps = starts[sel], lengths[sel], pos[sel]
+    st, ln, ps, kk = starts[sel], lengths[sel], pos[sel], k0[sel]
     R = sel.size
     row_base = np.concatenate([[0], np.cumsum(ln)[:-1]])
     row_of = np.repeat(np.arange(R), ln)
@@ -149,7 +151,7 @@ def bank_class_order(offs: np.ndarray, starts: np.ndarray, lengths: np.ndarray,
     rows = np.arange(R)
     for k in range(int(ln.max())):
         live = rows[ln > k]
-        c = (4 * (ps[live] % 8) + k) % 32
+        c = (4 * (ps[live] % 8) + kk[live] + k) % 32
         miss = np.nonzero(cnt[live, c] == taken[live, c])[0]
         if miss.size:  # target class used up: the next class up that is left
             lm = live[miss]
@@ -175,3 +177,15 @@ def length_sorted(member: np.ndarray, nop: np.ndarray, tile_particles: int,
             seg = member[i0:min(i0 + tile_particles, b)].copy()
             member[i0:i0 + seg.size] = seg[np.argsort(-nop[seg], kind="stable")]
     return member
+
+
+def cut_inner(q: np.ndarray, box: float, cl2: float, i: np.ndarray, j: np.ndarray) -> np.ndarray:
+    """The tile rows' cutoff classes: r2 <= cl2 with r2 = (dx*dx + dy*dy) +
+    dz*dz (separately rounded) over the threshold minimum image of
+    d = q_j - q_i (box > 0), as the library evaluates it."""
+    d = q[j, :3] - q[i, :3]
+    if box > 0:
+        h = 0.5 * box
+        d = np.where(d > h, d - box, np.where(d < -h, d + box, d))
+    r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    return r2 <= cl2

@@ -500,3 +500,45 @@ def test_chains_are_bitwise_one_chain(ctx, graphs, prec):
         np.testing.assert_array_equal(res[0], r)
     if prec == "f64":
         assert normwise(res[1][:, :3], oracle_momenta(12, 1.0, steps=106)) < FP64_TOL
+
+
+@pytest.mark.parametrize("prec", ["f64", G.ACCUM_F64])
+def test_cutoff_classes_stay_exact_when_particles_move(ctx, prec):
+    """The tile rows take their cutoff classes from the positions of the first
+    evaluation. Positions moved afterwards (some pairs cross the cutoff both
+    ways) still give the oracle's forces over the same list: every entry's r2
+    is evaluated at every step, and a batch is skipped only when all its
+    entries are beyond the cutoff at that step."""
+    q0, box, _, full = periodic(12, 1.0)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    dq = ctx.array(q0)
+    dp = ctx.zeros(q0.shape)
+    ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, 1)  # tiles and classes from q0
+    rng = np.random.default_rng(11)
+    q1 = q0.copy()
+    q1[:, :3] += rng.uniform(-0.12, 0.12, size=(q0.shape[0], 3))
+    q1[:, :3] %= box
+    # pairs of the list on both sides of the cutoff changed class
+    i = np.repeat(np.arange(full.n), full.number_of_partners)
+    j = full.sorted_list
+    from helpers import cut_inner
+    c0 = cut_inner(q0, box, G.CL2, i, j)
+    c1 = cut_inner(q1, box, G.CL2, i, j)
+    assert np.sum(c0 & ~c1) > 1000 and np.sum(~c0 & c1) > 1000
+    dq1 = ctx.array(q1)
+    for steps in (1, 40):
+        dp1 = ctx.zeros(q1.shape)
+        if prec == "f64":
+            ctx.force_aos_f64(dq1, dp1, 1.0, G.CL2, box, steps)
+        else:
+            ctx.force_aos_mixed(dq1, dp1, 1.0, G.CL2, box, steps, prec)
+        ctx.sync()
+        import oracle as O
+        F = np.zeros_like(q1)
+        O.force_aos(O.CSR(full.number_of_partners, full.pointer, full.sorted_list, "full"), q1,
+                    F, float(steps), O.CL2, box, method="full")
+        tol = FP64_TOL if prec == "f64" else MIXED_TOL[1.0]
+        assert normwise(dp1.download(), F) < tol

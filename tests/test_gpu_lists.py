@@ -226,7 +226,8 @@ def _check_spatial_tiles(ctx, q, box, full, tile):
     np.testing.assert_array_equal(
         member, length_sorted(column_order(q, box, tile), full.number_of_partners, tile))
     t = ctx.get_tiles()
-    h = G.make_tiles(full, tile, member)
+    # tiles derived with positions carry the cutoff classes of those positions
+    h = G.make_tiles(full, tile, member, q, box, G.CL2)
     assert t.max_stage == ctx.list_info().tile_max_stage
     # Identical whenever the host tiler did not have to shrink a block.
     if np.all(h.desc[:-1, 1] == tile):

@@ -317,7 +317,7 @@ def test_md_slot_order_copies_return_to_caller_order(ctx):
     assert order == G.TILE_ORDER_SPATIAL
     t = ctx.get_tiles()
     info = ctx.list_info()
-    h = G.make_tiles(full, info.tile_particles, member)
+    h = G.make_tiles(full, info.tile_particles, member, q0, box, G.CL2)
     assert np.all(h.desc[:-1, 1] == info.tile_particles)
     np.testing.assert_array_equal(t.desc, h.desc)
     np.testing.assert_array_equal(t.local, h.local)

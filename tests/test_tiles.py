@@ -7,7 +7,7 @@ import pytest
 
 import lj_gpu_amd as G
 import oracle as O
-from helpers import bank_class_order, column_order, length_sorted, normwise, periodic
+from helpers import bank_class_order, column_order, cut_inner, length_sorted, normwise, periodic
 
 MAX_STAGE = 65535 // 24
 
@@ -22,7 +22,7 @@ def slot_rows(csr, member):
     return mem, sptr
 
 
-def check_tiles(csr, t, tile_particles, member=None):
+def check_tiles(csr, t, tile_particles, member=None, q=None, box=0.0, cl2=O.CL2):
     n = csr.n
     mem, sptr = slot_rows(csr, member)
     desc = t.desc
@@ -57,12 +57,35 @@ def check_tiles(csr, t, tile_particles, member=None):
     np.testing.assert_array_equal(skey[np.minimum(at, skey.size - 1)], pkey)
     list_offs = (at - desc[b, 2]) * 24
     # Rows in the bank-class order (checked on about 200 evenly spaced blocks),
-    # each a permutation of its list row (checked on all).
+    # each a permutation of its list row (checked on all). With positions the
+    # rows are first split into cutoff classes (entries within the cutoff
+    # first, both parts in list order), each part then in the bank-class
+    # order with its classes continuing from the part's start.
     row_pos = np.arange(n) - desc[block_of, 0]
     pick = np.nonzero(block_of % max(1, desc.shape[0] // 200) == 0)[0]
-    expect = bank_class_order(list_offs, sptr[pick], nop_slot[pick], row_pos[pick])
     sel = np.repeat(sptr[pick] - np.cumsum(np.r_[0, nop_slot[pick]][:-1]), nop_slot[pick]) + \
         np.arange(int(nop_slot[pick].sum()))
+    if q is None:
+        expect = bank_class_order(list_offs, sptr[pick], nop_slot[pick], row_pos[pick])
+    else:
+        i_of = np.repeat(mem, nop_slot)
+        inner = cut_inner(q, box, cl2, i_of, partners)
+        parted = list_offs.copy()
+        n_in = np.zeros(n, np.int64)
+        for s_ in pick:
+            a, m = sptr[s_], nop_slot[s_]
+            if m > 256:
+                continue
+            f = inner[a: a + m]
+            n_in[s_] = f.sum()
+            parted[a: a + m] = np.concatenate([list_offs[a: a + m][f], list_offs[a: a + m][~f]])
+        starts = np.concatenate([sptr[pick], sptr[pick] + n_in[pick]])
+        lens = np.concatenate([n_in[pick], nop_slot[pick] - n_in[pick]])
+        pos = np.concatenate([row_pos[pick], row_pos[pick]])
+        k0 = np.concatenate([np.zeros(pick.size, np.int64), n_in[pick]])
+        expect = bank_class_order(parted, starts, lens, pos, k0)
+        # the picked rows hold entries of both classes
+        assert n_in[pick].min() > 0 and (nop_slot[pick] - n_in[pick]).min() > 0
     np.testing.assert_array_equal(t.local[sel], expect[sel])
     row_id = np.repeat(np.arange(n, dtype=np.int64), nop_slot) << 16
     np.testing.assert_array_equal(np.sort(row_id + t.local), np.sort(row_id + list_offs))
@@ -291,3 +314,68 @@ def test_length_sorted_hand_cases():
     assert list(length_sorted([4, 0, 3, 1, 2, 5], nop, 4)) == [4, 1, 0, 3, 2, 5]
     # ties keep their order, and blocks restart at each system boundary
     assert list(length_sorted([0, 1, 2, 3], np.array([3, 3, 5, 3]), 4, [0, 1, 4])) == [0, 2, 1, 3]
+
+
+@pytest.mark.parametrize("k,rho,tp,spatial", [(12, 1.0, 64, True), (8, 0.5, 32, True),
+                                              (12, 1.0, 64, False), (31, 1.0, 64, True)])
+def test_cutoff_classes_of_tile_rows(k, rho, tp, spatial):
+    """With positions, each tile row holds the entries within the cutoff
+    first, then those beyond it (each part in list order, then bank-class
+    ordered with its classes continuing from the part's start); the tiling
+    still re-expresses the list and gives the oracle's forces."""
+    q, box, _, full = periodic(k, rho)
+    member = column_order(q, box, tp) if spatial else None
+    t = G.make_tiles(full, tp, member, q, box, G.CL2)
+    check_tiles(full, t, tp, member, q, box, G.CL2)
+    if k <= 12:
+        F = emulate_tile_force(q, box, full, t, member=member)
+        p = np.zeros_like(q)
+        O.force_aos(O.CSR(full.number_of_partners, full.pointer, full.sorted_list, "full"),
+                    q, p, 1.0, O.CL2, box, method="full")
+        assert normwise(F, p[:, :3]) < 1e-13
+    # a wrong class boundary is caught: classes of another cutoff differ
+    if k == 12 and spatial:
+        t2 = G.make_tiles(full, tp, member, q, box, 2.8 ** 2)
+        assert not np.array_equal(t2.local, t.local)
+        with pytest.raises(AssertionError):
+            check_tiles(full, t2, tp, member, q, box, G.CL2)
+
+
+def test_cutoff_classes_hand_case():
+    """Four particles on a line in an open box: partners of particle 0 at
+    distances 1, 3.1 and 2 (list order 1, 2, 3); with cutoff 3 the row of
+    particle 0 becomes partners 1, 3 (within) then 2 (beyond)."""
+    q = np.zeros((4, 4))
+    q[:, 0] = [0.0, 1.0, 3.1, 2.0]
+    nop = np.array([3, 3, 3, 3], np.int32)
+    ptr = np.array([0, 3, 6, 9], np.int32)
+    lst = np.array([1, 2, 3, 0, 2, 3, 0, 1, 3, 0, 1, 2], np.int32)
+    csr = G.CSR(nop, ptr, lst, G.LIST_FULL)
+    t = G.make_tiles(csr, 64, None, q, 0.0, 9.0)
+    stage = t.stage[t.desc[0, 2]: t.desc[0, 2] + t.desc[0, 3]]
+    row0 = stage[t.local[0:3] // 24]
+    assert sorted(row0[:2].tolist()) == [1, 3] and row0[2] == 2
+    # without positions the row keeps all three partners in one class
+    t0 = G.make_tiles(csr, 64)
+    assert sorted(stage[t0.local[0:3] // 24].tolist()) == [1, 2, 3]
+
+
+def test_bank_class_order_continues_from_the_part_start():
+    row = np.array([0, 32, 1, 33]) * 24
+    # a part starting at entry 1 of its row targets classes 1, 2, 3, 4 (r = 0):
+    # class 1 first, then 2, 3 and 4 wrap round to class 0, then class 1 again
+    got = list(bank_class_order(row, [0], [4], [0], [1]) // 24)
+    assert got == [1, 0, 32, 33]
+    # the literal loop with the targets shifted by the part start agrees
+    rng = np.random.default_rng(3)
+    ranks = list(rng.choice(300, 40, replace=False))
+    for r, k0 in ((0, 0), (5, 17), (12, 33)):
+        pools = [[v for v in ranks if v % 32 == c] for c in range(32)]
+        want = []
+        for k in range(len(ranks)):
+            c = (4 * (r % 8) + k0 + k) % 32
+            while not pools[c]:
+                c = (c + 1) % 32
+            want.append(pools[c].pop(0))
+        got = bank_class_order(np.array(ranks) * 24, [0], [len(ranks)], [r], [k0]) // 24
+        assert list(got) == want
