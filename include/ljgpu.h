@@ -362,6 +362,23 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t particle_number, 
 int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
                  double* potential, double* kinetic, int64_t* pairs_in_cutoff);
 
+/* ljgpu_energy without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, checks and results as ljgpu_energy (the same doubles: the
+ * partials are added in the same order). Its partial sums live in a static
+ * per-GPU array of 3 * 1024 + 3 doubles (24.6 KB, held by every GPU the
+ * library runs on until the process ends) instead of context memory. The
+ * grid is capped at 1,024 workgroups that stride over the rows, so no
+ * particle count or list length needs more scratch than that. All calls on
+ * one GPU share the array: calls that overlap on one GPU (from several
+ * threads, or contexts on different streams) race and must be serialised by
+ * the caller. The list must already be installed (installing one
+ * allocates). Synchronous. */
+int ljgpu_energy_noalloc(ljgpu_ctx* ctx, const double* q, const double* p, double box,
+                         double cl2, double* potential, double* kinetic,
+                         int64_t* pairs_in_cutoff);
+
 /* Maxwell-Boltzmann momenta at `temperature` (unit masses): each component
  * N(0, temperature) from std::mt19937(seed) in particle order x, y, z, then
  * the mean momentum removed. Host AoS p[N][4] (pad set to 0). */

@@ -178,6 +178,8 @@ def lib() -> C.CDLL:
         "ljgpu_md_run": [_vp, _vp, _vp, _i32, _d, C.POINTER(MDConfig), _i32, C.POINTER(MDStats),
                          _vp, _i32],
         "ljgpu_energy": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i64)],
+        "ljgpu_energy_noalloc": [_vp, _vp, _vp, _d, _d, C.POINTER(_d), C.POINTER(_d),
+                                 C.POINTER(_i64)],
         "ljgpu_maxwell_boltzmann": [_i32, _d, C.c_uint32, _vp],
         "ljgpu_set_graphs": [_vp, C.c_int],
         "ljgpu_graph_stats": [_vp, C.POINTER(_i64), C.POINTER(_i32)],
@@ -633,6 +635,17 @@ class Context:
         u, k, c = _d(0.0), _d(0.0), _i64(0)
         _check(lib().ljgpu_energy(self.handle, q.ptr, p.ptr, box, cl2, C.byref(u), C.byref(k),
                                   C.byref(c)))
+        return u.value, k.value, c.value
+
+    def energy_pairs_noalloc(self, q: DeviceArray, p: DeviceArray, box: float = 0.0,
+                             cl2: float = CL2) -> tuple[float, float, int]:
+        """energy_pairs through ljgpu_energy_noalloc: no device allocation,
+        partial sums in the GPU's static scratch (calls overlapping on one GPU
+        race)."""
+        self._check_aos(q, p)
+        u, k, c = _d(0.0), _d(0.0), _i64(0)
+        _check(lib().ljgpu_energy_noalloc(self.handle, q.ptr, p.ptr, box, cl2, C.byref(u),
+                                          C.byref(k), C.byref(c)))
         return u.value, k.value, c.value
 
     def md_run(self, q: DeviceArray, p: DeviceArray, box: float, steps: int, dt: float = 0.005,

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -14,6 +15,36 @@
 using namespace ljgpu;
 
 namespace ljgpu {
+
+// Scratch of the allocation-free entry point ljgpu_energy_noalloc: every GPU
+// holds its own copy of this array, at its own address, until the process
+// ends. In host code the name is only a stand-in for the symbol, never a
+// device address: host code uses the address hipGetSymbolAddress returns for
+// the current device (noalloc_scratch_of), for kernel arguments and copies.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_addr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: looked
+// up on the first call for the device, then kept.
+int noalloc_scratch_of(int device, double** out) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return set_error(LJGPU_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_addr[device]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_addr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_addr[device];
+  return LJGPU_OK;
+}
+
+}  // namespace
 
 int use_device(ljgpu_ctx* ctx) {
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
@@ -655,6 +686,11 @@ int ljgpu_create(int device, ljgpu_ctx** out) {
   if (e != hipSuccess) {
     ljgpu_destroy(ctx);
     return hip_fail(e, "context creation");
+  }
+  const int rc = noalloc_scratch_of(device, &ctx->noalloc_scratch);
+  if (rc) {
+    ljgpu_destroy(ctx);
+    return rc;
   }
   *out = ctx;
   return LJGPU_OK;

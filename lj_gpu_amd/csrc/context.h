@@ -40,6 +40,15 @@ constexpr int32_t kMinChainTiles = 64;  // fewer tiles: one chain
 // Calls with at least kSlotMinSteps steps over spatially ordered tiles run
 // over copies of q and p in slot order (two gathers and a scatter per call).
 constexpr int32_t kSlotMinSteps = 16;
+// Energy sums: a fixed grid of kEnergyBlocks workgroups (each striding over
+// the rows) writes one partial per workgroup and quantity (potential,
+// kinetic, pairs inside the cutoff), added afterwards in block order, so the
+// result does not depend on scheduling or on the particle count.
+constexpr int kEnergyBlocks = 1024;
+// Doubles in the static per-GPU scratch of the allocation-free entry point
+// ljgpu_energy_noalloc: the 3 * kEnergyBlocks partials and the 3 totals
+// (24.6 KB, whatever the particle count).
+constexpr int kNoAllocScratchDoubles = 3 * kEnergyBlocks + 3;
 
 // Every argument of a captured force loop, written field by field into a
 // byte string (so padding inside the argument structs never enters it); a
@@ -134,6 +143,9 @@ struct ljgpu_ctx {
   // The loop's working copies of q and p in a spatial order, a staging
   // buffer, and the caller's index of each slot (two buffers).
   ljgpu::DevMem md_q, md_p, md_tmp, md_perm, md_perm2;
+  // Device address of this GPU's static scratch array (looked up once per
+  // device at context creation; shared by every context on the device).
+  double* noalloc_scratch = nullptr;
 
   int lanes = 4;
   int unroll = 4;

@@ -29,7 +29,6 @@ namespace ljgpu {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kEnergyBlocks = 1024;  // fixed grid: partial sums reduce in a fixed order
 
 // x mapped into [0, box): one multiply by 1/box instead of a divide, then
 // the last-bit cases (a result just below 0, or equal to box) folded in.
@@ -205,17 +204,30 @@ int grid_for(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65535)));
 }
 
-// U and K of the current state (synchronises; partial sums added in a fixed
-// order, so the result does not depend on scheduling).
-int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2, double* u,
-           double* k, double* pairs) {
-  *u = 0.0;
-  *k = 0.0;
-  *pairs = 0.0;
-  if (ctx->n == 0) return LJGPU_OK;
-  TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
-  double* part = ctx->md_red.as<double>();
+// Sums of the energy partials in block order b = 0, 1, ..., grid - 1 from
+// 0.0 (the order of energy()'s host loop, so both give the same doubles):
+// out = (U, K, pairs).
+__global__ void k_energy_total(int grid, int with_q, int with_p,
+                               const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double u = 0.0, k = 0.0, c = 0.0;
+  for (int b = 0; b < grid; b++) {
+    if (with_q) u += part[b];
+    if (with_p) k += part[kEnergyBlocks + b];
+    if (with_q) c += part[2 * kEnergyBlocks + b];
+  }
+  out[0] = u;
+  out[1] = k;
+  out[2] = c;
+}
+
+// Launch the energy kernels of the current state into `part` (3 *
+// kEnergyBlocks doubles: potential, kinetic and pair-count partials, one
+// per workgroup of the returned grid).
+int launch_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
+                  double* part, int* grid_out) {
   const int grid = std::min(kEnergyBlocks, grid_for(ctx->n));
+  *grid_out = grid;
   if (q) {
     const double weight = ctx->eval_half ? 1.0 : 0.5;
     k_potential<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, ctx->eval.nop.as<int32_t>(),
@@ -226,6 +238,21 @@ int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double 
   }
   if (p) k_kinetic<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, p, part + kEnergyBlocks);
   HIP_TRY(hipGetLastError(), "energy kernels");
+  return LJGPU_OK;
+}
+
+// U and K of the current state (synchronises; partial sums added in a fixed
+// order, so the result does not depend on scheduling).
+int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2, double* u,
+           double* k, double* pairs) {
+  *u = 0.0;
+  *k = 0.0;
+  *pairs = 0.0;
+  if (ctx->n == 0) return LJGPU_OK;
+  TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
+  double* part = ctx->md_red.as<double>();
+  int grid = 0;
+  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid));
   std::vector<double> h(3 * kEnergyBlocks);
   HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
                          ctx->stream), "hipMemcpyAsync");
@@ -258,6 +285,40 @@ int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, d
   double pairs = 0.0;
   TRY(energy(ctx, q, p, box, cl2, potential, kinetic, &pairs));
   if (pairs_in_cutoff) *pairs_in_cutoff = static_cast<int64_t>(pairs);
+  return LJGPU_OK;
+}
+
+int ljgpu_energy_noalloc(ljgpu_ctx* ctx, const double* q, const double* p, double box,
+                         double cl2, double* potential, double* kinetic,
+                         int64_t* pairs_in_cutoff) {
+  static_assert(kNoAllocScratchDoubles >= 3 * kEnergyBlocks + 3, "scratch too small");
+  if (!ctx || !potential || !kinetic) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->kind < 0) return set_error(LJGPU_ERR_NO_LIST, "no neighbour list installed");
+  if (!ctx->rep_off.empty())
+    return set_error(LJGPU_ERR_UNSUPPORTED, "energy of batched replicas");
+  if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
+  if (!(cl2 >= 0.0)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "cl2 must be >= 0");
+  if (!ctx->noalloc_scratch) return set_error(LJGPU_ERR_HIP, "no scratch address");
+  TRY(use_device(ctx));
+  *potential = 0.0;
+  *kinetic = 0.0;
+  if (pairs_in_cutoff) *pairs_in_cutoff = 0;
+  if (ctx->n == 0) return LJGPU_OK;
+  // Every partial the totals read is written by this call's kernels (one per
+  // workgroup of the grid), so nothing in the scratch needs clearing.
+  double* part = ctx->noalloc_scratch;
+  double* total = part + 3 * kEnergyBlocks;
+  int grid = 0;
+  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid));
+  k_energy_total<<<1, 64, 0, ctx->stream>>>(grid, q != nullptr, p != nullptr, part, total);
+  HIP_TRY(hipGetLastError(), "k_energy_total");
+  double h[3] = {0.0, 0.0, 0.0};
+  HIP_TRY(hipMemcpyAsync(h, total, sizeof(h), hipMemcpyDeviceToHost, ctx->stream),
+          "hipMemcpyAsync");
+  HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  *potential = h[0];
+  *kinetic = h[1];
+  if (pairs_in_cutoff) *pairs_in_cutoff = static_cast<int64_t>(h[2]);
   return LJGPU_OK;
 }
 
