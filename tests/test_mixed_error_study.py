@@ -1,0 +1,200 @@
+"""Error study of mixed-precision force forms (CPU, numpy emulation).
+
+Workload: FCC 4*31^3 = 119,164 particles, densities 1.0 and 0.5, periodic,
+full list (cutoff 3.0, search 3.3). Reference: the oracle's FP64 force over
+the same list. Error measure: normwise max|dF| / max|F| (and RMS), the
+measure of the mixed-precision bound (1e-6 at density 1.0).
+
+Forms, each with FP32 force arithmetic as the mixed kernel does it
+(r^6 = r2^3, r^14 = r^6 r^6 r2, df = (24 r^6 - 48) / r^14; every FP32
+operation correctly rounded, a fused multiply-add rounded once):
+
+- "current": the shipped mixed kernel. Differences and r2 in FP64 from the
+  FP64 positions, r2 rounded once to FP32, df in FP32, F += df * d in FP64
+  (or d rounded to FP32 and Kahan-compensated FP32 sums).
+- "fp32_tile": positions relative to their tile's reference point rounded
+  to FP32 (12-byte LDS entries), differences, r2 and F in FP32 arithmetic,
+  FP64 accumulation.
+- "int32_tile": positions relative to the tile's reference point in int32
+  fixed point (2^-27 units; exact integer differences), each difference
+  rounded once to FP32, then as fp32_tile.
+- "int32_exact_r2": as int32_tile with r2 summed exactly in int64 and
+  rounded once to FP32, and F += df * d with the exact difference in FP64.
+
+Tiles are the cubes of a grid of side about cbrt(64 / density) (64
+particles at density 1), the reference point the cube's centre; the
+largest tile-relative coordinate is about 5.3 at density 1.0.
+
+Unless a form says otherwise, the pair set is the FP64 one (r2 <= 9 in
+FP64). Deciding the cutoff on a form's own FP32 r2 instead flips the few
+pairs within an FP32 rounding of r = 3, and each flip moves a force by
+|f(3)| * 3 ~ 0.011 (the fp32_tile "own cutoff" row).
+
+Outcome (asserted below): at density 1.0 the FP32 tile-relative form
+misses 1e-6 (about 2.7e-6); the forms with exact differences meet it: the
+current form (about 4.9e-7), int32 fixed point with exact r2 (about 6.2e-7)
+and, with less margin, int32 fixed point with FP32 r2 (about 9.1e-7).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import oracle_force, periodic
+
+BOUND = {1.0: 1.0e-6, 0.5: 2.0e-6}
+FIX_BITS = 27
+F32 = np.float32
+
+
+def _fma32(a, b, c):
+    """fmaf: the FP32 product is exact in FP64, the sum is rounded once
+    (to FP64, then FP32; the double rounding matters only at exact FP32
+    midpoints, which these data do not hit measurably)."""
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(F32)
+
+
+def _df32(r2):
+    r6 = r2 * r2 * r2
+    r14 = r6 * r6 * r2
+    return _fma32(np.full_like(r6, 24.0), r6, np.full_like(r6, -48.0)) * (F32(1.0) / r14)
+
+
+def _r2_32(d):
+    return _fma32(d[:, 2], d[:, 2], _fma32(d[:, 1], d[:, 1], d[:, 0] * d[:, 0]))
+
+
+class _Setup:
+    def __init__(self, k, rho):
+        q, box, _, full = periodic(k, rho)
+        self.F = oracle_force(k, rho)[:, :3]
+        n = q.shape[0]
+        self.n = n
+        nop = full.number_of_partners
+        self.nop = nop
+        self.i = np.repeat(np.arange(n), nop)
+        j = full.sorted_list.astype(np.int64)
+        x = q[:, :3]
+        ncell = max(1, int(round(box / np.cbrt(64.0 / rho))))
+        a = box / ncell
+        c = (np.floor(x / a) + 0.5) * a
+        ci = c[self.i]
+        dj = x[j] - ci
+        dj -= box * np.rint(dj / box)
+        di = x[self.i] - ci
+        di -= box * np.rint(di / box)
+        self.dj, self.di = dj, di
+        d = x[j] - x[self.i]
+        d -= box * np.rint(d / box)
+        self.d64 = d
+        self.r2_64 = (d * d).sum(axis=1)
+        self.inside = self.r2_64 <= 9.0
+        self.max_rel = float(np.abs(np.concatenate([dj, di])).max())
+
+    def acc64(self, df, d):
+        contrib = df.astype(np.float64)[:, None] * d.astype(np.float64)
+        return np.stack([np.bincount(self.i, weights=contrib[:, c], minlength=self.n)
+                         for c in range(3)], axis=1)
+
+    def acc_kahan32(self, df, d32):
+        """Per-row Kahan-compensated FP32 sums in list order (rows advanced
+        together, one entry column at a time)."""
+        contrib = df[:, None] * d32
+        start = np.zeros(self.n, np.int64)
+        start[1:] = np.cumsum(self.nop)[:-1]
+        s = np.zeros((self.n, 3), F32)
+        comp = np.zeros((self.n, 3), F32)
+        for k in range(int(self.nop.max())):
+            rows = np.nonzero(self.nop > k)[0]
+            v = contrib[start[rows] + k]
+            y = v - comp[rows]
+            t = s[rows] + y
+            comp[rows] = (t - s[rows]) - y
+            s[rows] = t
+        return s.astype(np.float64) - comp.astype(np.float64)
+
+    def err(self, Fm):
+        sc = np.abs(self.F).max()
+        return (float(np.abs(Fm - self.F).max() / sc),
+                float(np.sqrt(((Fm - self.F) ** 2).mean()) / sc))
+
+    def forms(self):
+        out = {}
+        # current: FP64 differences and r2, r2 rounded once to FP32
+        df = np.where(self.inside, _df32(self.r2_64.astype(F32)), F32(0))
+        out["current, FP64 accumulation"] = self.err(self.acc64(df, self.d64))
+        out["current, Kahan FP32 accumulation"] = self.err(
+            self.acc_kahan32(df, self.d64.astype(F32)))
+        # FP32 tile-relative positions
+        d32 = self.dj.astype(F32) - self.di.astype(F32)
+        r2 = _r2_32(d32)
+        out["fp32_tile"] = self.err(self.acc64(np.where(self.inside, _df32(r2), F32(0)), d32))
+        own = r2 <= F32(9.0)
+        out["fp32_tile, own cutoff"] = self.err(self.acc64(np.where(own, _df32(r2), F32(0)), d32))
+        out["fp32_tile, own cutoff, flipped pairs"] = (int((own != self.inside).sum()), 0.0)
+        # int32 fixed point, tile-relative
+        s = 2.0 ** FIX_BITS
+        qi = np.rint(self.di * s).astype(np.int64)
+        qj = np.rint(self.dj * s).astype(np.int64)
+        assert np.abs(qi).max() < 2 ** 31 and np.abs(qj).max() < 2 ** 31
+        dint = qj - qi
+        d32 = dint.astype(F32) * F32(1.0 / s)
+        r2 = _r2_32(d32)
+        out["int32_tile"] = self.err(self.acc64(np.where(self.inside, _df32(r2), F32(0)), d32))
+        r2i = (dint * dint).sum(axis=1)  # exact: |dint| < 2^31
+        r2e = (r2i.astype(np.float64) / (s * s)).astype(F32)
+        flips = int(((r2i <= int(9 * s * s)) != self.inside).sum())
+        out["int32_exact_r2"] = self.err(self.acc64(np.where(self.inside, _df32(r2e), F32(0)),
+                                                    dint.astype(np.float64) / s))
+        out["int32_exact_r2, flipped pairs"] = (flips, 0.0)
+        return out
+
+
+@pytest.fixture(scope="module", params=[1.0, 0.5])
+def study(request):
+    rho = request.param
+    st = _Setup(31, rho)
+    res = st.forms()
+    print(f"\nmixed-precision error study, N={st.n}, density {rho}, largest tile-relative "
+          f"coordinate {st.max_rel:.2f}")
+    for name, (mx, rms) in res.items():
+        if "flipped" in name:
+            print(f"  {name:42s} {mx}")
+        else:
+            print(f"  {name:42s} max {mx:.3e}  rms {rms:.3e}")
+    return rho, res
+
+
+def test_current_form_meets_the_bound(study):
+    rho, res = study
+    assert res["current, FP64 accumulation"][0] < BOUND[rho]
+    assert res["current, Kahan FP32 accumulation"][0] < BOUND[rho]
+
+
+def test_exact_integer_r2_meets_the_bound(study):
+    rho, res = study
+    assert res["int32_exact_r2"][0] < BOUND[rho]
+    assert res["int32_exact_r2, flipped pairs"][0] == 0
+
+
+def test_twelve_byte_forms(study):
+    """12-byte positions: FP32 tile-relative coordinates miss the bound at
+    density 1.0; int32 fixed-point coordinates (exact differences, each
+    rounded once to FP32, fused r2) stay inside it, with less margin than
+    the current form."""
+    rho, res = study
+    cur = res["current, FP64 accumulation"][0]
+    if rho == 1.0:
+        assert res["fp32_tile"][0] > BOUND[rho]
+        # A flipped cutoff pair costs far more than the arithmetic.
+        assert res["fp32_tile, own cutoff, flipped pairs"][0] > 0
+        assert res["fp32_tile, own cutoff"][0] > 10 * res["fp32_tile"][0]
+    assert cur < res["int32_tile"][0] < BOUND[rho]
+    assert res["fp32_tile"][0] > 2 * cur
+
+
+def test_error_ordering(study):
+    """Each step away from exact differences adds error."""
+    _, res = study
+    cur = res["current, FP64 accumulation"][1]
+    assert cur <= res["int32_tile"][1] <= res["fp32_tile"][1]
