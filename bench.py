@@ -257,22 +257,33 @@ def bench_replicas(args, G, np) -> dict:
 def bench_md(args, G, np) -> dict:
     """Velocity Verlet on FCC 4*128^3 = 8,388,608 particles at density 0.8,
     Maxwell-Boltzmann momenta at T = 1 (seed 7), dt 0.005, mixed-precision
-    forces, the list rebuilt on the device whenever a particle has moved half
-    the skin (0.8 by default) since the last build, so no step runs on a
-    stale list.
-    Throughput is particle-steps per second; list entries per second (which
-    grow with the skin) are reported beside it."""
+    forces, the list rebuilt on the device every --md-every steps (skin
+    --md-skin), or, with --md-every 0 (default, skin 0.8), whenever a
+    particle has moved half the skin since the last build, so no step runs
+    on a stale list.
+    Throughput is particle-steps per second. Beside it: pairs inside the
+    cutoff per second (the pair count of the start and end states, averaged,
+    times the steps, over the run's GPU time) and list entries per second
+    (which grow with the skin)."""
     q, box = G.lattice_periodic(128, 0.8, 2)
     p = G.maxwell_boltzmann(q.shape[0], 1.0, 7)
     ctx = G.Context(0)
     dq, dp = ctx.array(q), ctx.array(p)
-    steps = args.steps
-    skin = args.md_skin
-    st, samples = ctx.md_run(dq, dp, box, steps, 0.005, skin=skin, rebuild_every=0,
-                             precision=G.PREC_MIXED, sample_every=max(steps // 10, 1))
-    ctx.close()
     n = q.shape[0]
+
+    def pairs_in_cutoff():
+        ctx.build_list(dq, n, box, G.LIST_FULL)
+        return ctx.energy_pairs(dq, dp, box)[2]
+
+    pairs0 = pairs_in_cutoff()
+    steps, skin, every = args.steps, args.md_skin, args.md_every
+    st, samples = ctx.md_run(dq, dp, box, steps, 0.005, skin=skin, rebuild_every=every,
+                             precision=G.PREC_MIXED, sample_every=max(steps // 10, 1))
+    pairs1 = pairs_in_cutoff()
+    ctx.close()
     total_s = st.ms_total / 1e3
+    policy = (f"every {every} steps" if every > 0 else
+              "when a particle has moved skin/2")
     return {
         "metric": "particle-timesteps/s, LJ velocity-Verlet MD", "value":
             round(n * steps / total_s / 1e9, 4),
@@ -281,8 +292,11 @@ def bench_md(args, G, np) -> dict:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64-mixed", "data": "synthetic",
         "config": {"workload": f"LJ velocity Verlet, FCC 4*128^3={n} particles, density 0.8, "
                                f"periodic box {box:.4f}, T=1 Maxwell-Boltzmann (seed 7), dt 0.005, "
-                               f"device list rebuilt when a particle has moved skin/2 (skin "
-                               f"{skin}), mixed forces, FP64 integration", "particles": n},
+                               f"device list rebuilt {policy} (skin {skin}), mixed forces, FP64 "
+                               f"integration", "particles": n, "rebuild_every": every,
+                   "skin": skin},
+        "pairs_in_cutoff_per_s": round(0.5 * (pairs0 + pairs1) * steps / total_s / 1e9, 3),
+        "pairs_in_cutoff": [pairs0, pairs1],
         "list_entries_per_s": round(st.pair_entries / total_s / 1e9, 3),
         "md": {"rebuilds": st.rebuilds, "ms_rebuild": st.ms_rebuild, "ms_total": st.ms_total,
                "ms_per_rebuild": st.ms_rebuild / max(st.rebuilds, 1),
@@ -333,7 +347,10 @@ def main() -> int:
                          "N=1,048,576; replicas: 256 batched boxes; md: 8.4M-particle velocity "
                          "Verlet with device list rebuilds")
     ap.add_argument("--md-skin", type=float, default=0.8,
-                    help="md workload: list skin (rebuild when a particle moved skin/2)")
+                    help="md workload: list skin (search length = cutoff + skin)")
+    ap.add_argument("--md-every", type=int, default=0,
+                    help="md workload: rebuild the list every this many steps (0: whenever a "
+                         "particle has moved skin/2 since the last build)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step individually instead of replaying captured graphs "
                          "(for per-dispatch profiler counters)")

@@ -163,6 +163,10 @@ int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t particle_number, d
                      double search_length, int kind);
 int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
                          int32_t particle_number, double box, double search_length, int kind);
+/* The same from packed double[N][3] positions (8-byte aligned), the
+ * reference's double3 buffers (cuda/force_cuda.cu:26-27). */
+int ljgpu_build_list_aos3(ljgpu_ctx* ctx, const double* q, int32_t particle_number, double box,
+                          double search_length, int kind);
 
 /* Batched replicas: n_replicas independent systems stored back to back,
  * replica r holding particles [offsets[r], offsets[r+1]) in a cubic box of
@@ -277,6 +281,15 @@ int ljgpu_force_soa_f64(ljgpu_ctx* ctx, const double* qx, const double* qy, cons
                         double* px, double* py, double* pz, double dt, double cl2,
                         double box, int32_t steps);
 
+/* The same over packed double[N][3] arrays (x, y, z; 24 B per particle,
+ * 8-byte aligned): the reference's double3 instantiation of every kernel
+ * (cuda/force_cuda.cu:26-27 buffers, :354-375 MEASURE_FOR_ALLTYPES). Same
+ * arithmetic and results as the double4 calls. A call of 16 or more steps
+ * over spatially ordered tiles runs on internal 4-double copies in slot
+ * order, like the double4 calls, and writes p back in the caller's rows. */
+int ljgpu_force_aos3_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                         double box, int32_t steps);
+
 /* Mixed precision: pair distances and r2 in FP64, r2 rounded once to FP32,
  * r^-6 / r^-14 and the force scalar in FP32, accumulation in FP64 or
  * Kahan-compensated FP32 (accum = LJGPU_ACCUM_*), momentum update in FP64.
@@ -287,6 +300,8 @@ int ljgpu_force_aos_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt,
 int ljgpu_force_soa_mixed(ljgpu_ctx* ctx, const double* qx, const double* qy,
                           const double* qz, double* px, double* py, double* pz, double dt,
                           double cl2, double box, int32_t steps, int accum);
+int ljgpu_force_aos3_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                           double box, int32_t steps, int accum);
 
 /* One launch with exactly the reference kernel's argument list
  * (force_kernel_*(q, p, particle_number, dt, CL2, sorted_list,

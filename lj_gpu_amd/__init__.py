@@ -149,6 +149,9 @@ def lib() -> C.CDLL:
         "ljgpu_set_tiling": [_vp, _i32, C.c_int, _i32],
         "ljgpu_set_kernel_config": [_vp, C.c_int, C.c_int, C.c_int],
         "ljgpu_force_aos_f64": [_vp, _vp, _vp, _d, _d, _d, _i32],
+        "ljgpu_force_aos3_f64": [_vp, _vp, _vp, _d, _d, _d, _i32],
+        "ljgpu_force_aos3_mixed": [_vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
+        "ljgpu_build_list_aos3": [_vp, _vp, _i32, _d, _d, C.c_int],
         "ljgpu_force_soa_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32],
         "ljgpu_force_aos_mixed": [_vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
         "ljgpu_force_soa_mixed": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _d, _i32, C.c_int],
@@ -547,6 +550,12 @@ class Context:
         _need(q, 32 * n, "q (AoS double4)")
         _check(lib().ljgpu_build_list(self.handle, q.ptr, n, box, search_length, kind))
 
+    def build_list_aos3(self, q: DeviceArray, n: int, box: float = 0.0, kind: int = LIST_HALF,
+                        search_length: float = SEARCH_LENGTH) -> None:
+        """build_list from packed double[N][3] device positions."""
+        _need(q, 24 * n, "q (AoS double3)")
+        _check(lib().ljgpu_build_list_aos3(self.handle, q.ptr, n, box, search_length, kind))
+
     def build_list_soa(self, q3: list, n: int, box: float = 0.0, kind: int = LIST_HALF,
                        search_length: float = SEARCH_LENGTH) -> None:
         for c in range(3):
@@ -703,6 +712,23 @@ class Context:
                       box: float = 0.0, steps: int = 1) -> None:
         self._check_aos(q, p)
         _check(lib().ljgpu_force_aos_f64(self.handle, q.ptr, p.ptr, dt, cl2, box, steps))
+
+    def _check_aos3(self, q: DeviceArray, p: DeviceArray) -> None:
+        n = self._particles()
+        _need(q, 24 * n, "q (AoS double3)")
+        _need(p, 24 * n, "p (AoS double3)")
+
+    def force_aos3_f64(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
+                       box: float = 0.0, steps: int = 1) -> None:
+        """force_aos_f64 over packed double[N][3] arrays."""
+        self._check_aos3(q, p)
+        _check(lib().ljgpu_force_aos3_f64(self.handle, q.ptr, p.ptr, dt, cl2, box, steps))
+
+    def force_aos3_mixed(self, q: DeviceArray, p: DeviceArray, dt: float, cl2: float = CL2,
+                         box: float = 0.0, steps: int = 1, accum: int = ACCUM_F64) -> None:
+        self._check_aos3(q, p)
+        _check(lib().ljgpu_force_aos3_mixed(self.handle, q.ptr, p.ptr, dt, cl2, box, steps,
+                                            accum))
 
     def force_soa_f64(self, q3: list, p3: list, dt: float, cl2: float = CL2, box: float = 0.0,
                       steps: int = 1) -> None:

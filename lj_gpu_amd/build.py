@@ -28,7 +28,9 @@ ARCH = os.environ.get("LJGPU_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["api.hip", "lists.hip", "md.hip", "tune.hip", "force_f64.hip", "force_mixed.hip", "force_kahan.hip"]
+HIP_SOURCES = ["api.hip", "lists.hip", "md.hip", "tune.hip", "force_f64.hip", "force_mixed.hip",
+               "force_kahan.hip", "force_aos3_f64.hip", "force_aos3_mixed.hip",
+               "force_aos3_kahan.hip"]
 HOST_SOURCES = ["host.cpp", "paircache.cpp"]
 
 HIP_FLAGS = [

@@ -178,7 +178,7 @@ PosIn positions_of(const ForceArgs& fa) {
     in.x = fa.q[0];
     in.y = fa.q[0] + 1;
     in.z = fa.q[0] + 2;
-    in.stride = 4;
+    in.stride = fa.aos3 ? 3 : 4;
   }
   return in;
 }
@@ -306,6 +306,7 @@ int build_list(ljgpu_ctx* ctx, const PosIn& q, int32_t n, double box, double sea
 }
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7u) == 0; }
 
 int check_force_args(ljgpu_ctx* ctx, double dt, double cl2, int32_t steps, double box) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
@@ -372,6 +373,10 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
 
 int launch_one_on(hipStream_t stream, const ListView& lv, const LaunchConfig& lc,
                   const ForceArgs& fa, bool mixed, int accum) {
+  if (fa.aos3)
+    return !mixed ? launch_force_aos3_f64(stream, lv, fa, lc)
+           : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_aos3_mixed_kahan(stream, lv, fa, lc)
+                                            : launch_force_aos3_mixed_f64acc(stream, lv, fa, lc);
   return !mixed ? launch_force_f64(stream, lv, fa, lc)
          : accum == LJGPU_ACCUM_KAHAN_F32 ? launch_force_mixed_kahan(stream, lv, fa, lc)
                                           : launch_force_mixed_f64acc(stream, lv, fa, lc);
@@ -420,6 +425,7 @@ StepKey step_key(const ForceArgs& fa, const ListView& lv, const LaunchConfig& lc
                  int accum, int32_t steps, int32_t chains) {
   StepKey k;
   k.add(fa.soa);
+  k.add(fa.aos3);
   for (int c = 0; c < 3; c++) {
     k.add(fa.q[c]);
     k.add(fa.p[c]);
@@ -528,6 +534,26 @@ __global__ void k_slot_copy_soa(int32_t n, const int32_t* __restrict__ member, c
   }
 }
 
+// Packed double[N][3] rows to and from 4-double slot rows (pad 0).
+template <bool GATHER>
+__global__ void k_slot_copy_aos3(int32_t n, const int32_t* __restrict__ member,
+                                 const double* __restrict__ in, double* __restrict__ out) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t i = member[t];
+    if (GATHER) {
+      const double* r = in + 3 * i;
+      reinterpret_cast<double4*>(out)[t] = make_double4(r[0], r[1], r[2], 0.0);
+    } else {
+      const double4 v = reinterpret_cast<const double4*>(in)[t];
+      double* r = out + 3 * i;
+      r[0] = v.x;
+      r[1] = v.y;
+      r[2] = v.z;
+    }
+  }
+}
+
 int slot_grid(int32_t n) { return static_cast<int>(std::min<int64_t>((n + 255) / 256, 4096)); }
 
 // Positions and momenta of `fa` copied into the context's slot-order
@@ -541,7 +567,14 @@ int slot_gather(ljgpu_ctx* ctx, const ForceArgs& fa, ForceArgs* out) {
   *out = fa;
   double* q = ctx->slot_q.as<double>();
   double* p = ctx->slot_p.as<double>();
-  if (!fa.soa) {
+  if (fa.aos3) {
+    // The slot copies are 4-double rows, so the loop runs the AoS kernels.
+    k_slot_copy_aos3<true><<<slot_grid(n), 256, 0, ctx->stream>>>(n, mem, fa.q[0], q);
+    k_slot_copy_aos3<true><<<slot_grid(n), 256, 0, ctx->stream>>>(n, mem, fa.p[0], p);
+    out->aos3 = false;
+    out->q[0] = q;
+    out->p[0] = p;
+  } else if (!fa.soa) {
     k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
         n, mem, reinterpret_cast<const double4*>(fa.q[0]), reinterpret_cast<double4*>(q));
     k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
@@ -566,7 +599,9 @@ int slot_gather(ljgpu_ctx* ctx, const ForceArgs& fa, ForceArgs* out) {
 int slot_scatter(ljgpu_ctx* ctx, const ForceArgs& slot, const ForceArgs& fa) {
   const int32_t n = ctx->n;
   const int32_t* mem = ctx->tiles.member.as<int32_t>();
-  if (!fa.soa)
+  if (fa.aos3)
+    k_slot_copy_aos3<false><<<slot_grid(n), 256, 0, ctx->stream>>>(n, mem, slot.p[0], fa.p[0]);
+  else if (!fa.soa)
     k_slot_copy_aos<false><<<slot_grid(n), 256, 0, ctx->stream>>>(
         n, mem, reinterpret_cast<const double4*>(slot.p[0]), reinterpret_cast<double4*>(fa.p[0]));
   else
@@ -1064,6 +1099,18 @@ int ljgpu_build_list(ljgpu_ctx* ctx, const double* q, int32_t n, double box,
   return build_list(ctx, in, n, box, search_length, kind);
 }
 
+int ljgpu_build_list_aos3(ljgpu_ctx* ctx, const double* q, int32_t n, double box,
+                          double search_length, int kind) {
+  if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
+  if (n > 0 && !aligned8(q)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q must be 8-byte aligned");
+  PosIn in;
+  in.x = q;
+  in.y = q ? q + 1 : nullptr;
+  in.z = q ? q + 2 : nullptr;
+  in.stride = 3;
+  return build_list(ctx, in, n, box, search_length, kind);
+}
+
 int ljgpu_build_list_soa(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,
                          int32_t n, double box, double search_length, int kind) {
   if (!ctx) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null context");
@@ -1089,6 +1136,40 @@ int ljgpu_force_aos_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, d
   fa.cl2 = cl2;
   fa.box = box;
   return run_steps(ctx, fa, steps, false, 0);
+}
+
+int ljgpu_force_aos3_f64(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                         double box, int32_t steps) {
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
+  if (rc) return rc;
+  if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
+  if (!aligned8(q) || !aligned8(p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q and p must be 8-byte aligned");
+  ForceArgs fa;
+  fa.aos3 = true;
+  fa.q[0] = q;
+  fa.p[0] = p;
+  fa.dt = dt;
+  fa.cl2 = cl2;
+  fa.box = box;
+  return run_steps(ctx, fa, steps, false, 0);
+}
+
+int ljgpu_force_aos3_mixed(ljgpu_ctx* ctx, const double* q, double* p, double dt, double cl2,
+                           double box, int32_t steps, int accum) {
+  int rc = check_force_args(ctx, dt, cl2, steps, box);
+  if (rc) return rc;
+  if (accum != LJGPU_ACCUM_F64 && accum != LJGPU_ACCUM_KAHAN_F32)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "unknown accumulator");
+  if (ctx->n > 0 && (!q || !p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "null q or p");
+  if (!aligned8(q) || !aligned8(p)) return set_error(LJGPU_ERR_INVALID_ARGUMENT, "q and p must be 8-byte aligned");
+  ForceArgs fa;
+  fa.aos3 = true;
+  fa.q[0] = q;
+  fa.p[0] = p;
+  fa.dt = dt;
+  fa.cl2 = cl2;
+  fa.box = box;
+  return run_steps(ctx, fa, steps, true, accum);
 }
 
 int ljgpu_force_soa_f64(ljgpu_ctx* ctx, const double* qx, const double* qy, const double* qz,

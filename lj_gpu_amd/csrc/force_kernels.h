@@ -738,19 +738,26 @@ int run_layout(hipStream_t s, P pos, M mom, const ListView& lv, const PairParams
                         : run_lanes<PREC, P, M, false, false, false>(s, pos, mom, lv, pp, lc);
 }
 
+inline int pair_params(const ForceArgs& fa, const LaunchConfig& lc, PairParams* pp) {
+  if (lc.block < 64 || lc.block > 1024 || (lc.block % 64) != 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "block size must be a multiple of 64 in [64, 1024]");
+  pp->c24 = 24.0 * fa.dt;
+  pp->c48 = 48.0 * fa.dt;
+  pp->cl2 = fa.cl2;
+  pp->box = fa.box > 0.0 ? fa.box : 0.0;
+  pp->inv_box = fa.box > 0.0 ? 1.0 / fa.box : 0.0;
+  pp->c24f = static_cast<float>(pp->c24);
+  pp->c48f = static_cast<float>(pp->c48);
+  return LJGPU_OK;
+}
+
+// AoS (double[N][4]) or SoA arrays.
 template <int PREC>
 int run_force(void* stream, const ListView& lv, const ForceArgs& fa, const LaunchConfig& lc) {
   if (lv.n <= 0) return LJGPU_OK;
-  if (lc.block < 64 || lc.block > 1024 || (lc.block % 64) != 0)
-    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "block size must be a multiple of 64 in [64, 1024]");
   PairParams pp;
-  pp.c24 = 24.0 * fa.dt;
-  pp.c48 = 48.0 * fa.dt;
-  pp.cl2 = fa.cl2;
-  pp.box = fa.box > 0.0 ? fa.box : 0.0;
-  pp.inv_box = fa.box > 0.0 ? 1.0 / fa.box : 0.0;
-  pp.c24f = static_cast<float>(pp.c24);
-  pp.c48f = static_cast<float>(pp.c48);
+  const int rc = pair_params(fa, lc, &pp);
+  if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (fa.soa) {
     PosSoa pos{fa.q[0], fa.q[1], fa.q[2]};
@@ -760,6 +767,19 @@ int run_force(void* stream, const ListView& lv, const ForceArgs& fa, const Launc
   PosAos pos{fa.q[0]};
   MomAos mom{fa.p[0]};
   return run_layout<PREC>(s, pos, mom, lv, pp, lc);
+}
+
+// Packed AoS (double[N][3]) arrays.
+template <int PREC>
+int run_force_aos3(void* stream, const ListView& lv, const ForceArgs& fa,
+                   const LaunchConfig& lc) {
+  if (lv.n <= 0) return LJGPU_OK;
+  PairParams pp;
+  const int rc = pair_params(fa, lc, &pp);
+  if (rc) return rc;
+  PosAos3 pos{fa.q[0]};
+  MomAos3 mom{fa.p[0]};
+  return run_layout<PREC>(static_cast<hipStream_t>(stream), pos, mom, lv, pp, lc);
 }
 
 }  // namespace detail

@@ -1,7 +1,8 @@
 // lj_device.h — device-side building blocks shared by the force kernels.
 //
 // Layouts: AoS positions/momenta are double[N][4] (x, y, z, pad; 32 B per
-// particle, the reference's double4), SoA three double[N] arrays.
+// particle, the reference's double4), packed AoS double[N][3] (the
+// reference's double3), SoA three double[N] arrays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +40,20 @@ struct PosAos {
   }
 };
 
+// Packed AoS: double[N][3] (x, y, z; 24 B per particle, the reference's
+// double3). A row is only 8-byte aligned, so each coordinate is its own
+// 8-byte load; 32-bit byte offsets as above (valid for N < 2^32 / 24).
+struct PosAos3 {
+  const double* __restrict__ q;
+  __device__ __forceinline__ void ld(int j, double& x, double& y, double& z) const {
+    const uint32_t off = static_cast<uint32_t>(j) * 24u;
+    const char* base = reinterpret_cast<const char*>(q);
+    x = *reinterpret_cast<const double*>(base + off);
+    y = *reinterpret_cast<const double*>(base + off + 8u);
+    z = *reinterpret_cast<const double*>(base + off + 16u);
+  }
+};
+
 struct PosSoa {
   const double* __restrict__ x;
   const double* __restrict__ y;
@@ -63,6 +78,22 @@ struct MomAos {
   }
   __device__ __forceinline__ void atomic_add(int i, double ax, double ay, double az) const {
     double* r = p + 4 * static_cast<int64_t>(i);
+    unsafeAtomicAdd(r, ax);
+    unsafeAtomicAdd(r + 1, ay);
+    unsafeAtomicAdd(r + 2, az);
+  }
+};
+
+struct MomAos3 {
+  double* __restrict__ p;
+  __device__ __forceinline__ void add(int i, double ax, double ay, double az) const {
+    double* r = p + 3 * static_cast<int64_t>(i);
+    r[0] += ax;
+    r[1] += ay;
+    r[2] += az;
+  }
+  __device__ __forceinline__ void atomic_add(int i, double ax, double ay, double az) const {
+    double* r = p + 3 * static_cast<int64_t>(i);
     unsafeAtomicAdd(r, ax);
     unsafeAtomicAdd(r + 1, ay);
     unsafeAtomicAdd(r + 2, az);

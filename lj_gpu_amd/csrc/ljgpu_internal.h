@@ -216,7 +216,8 @@ bool build_tiles(const HostCSR& full, int32_t tile_particles, const int32_t* mem
 
 struct ForceArgs {
   bool soa = false;
-  const double* q[3] = {nullptr, nullptr, nullptr};  // AoS: q[0] is double[N][4]
+  bool aos3 = false;  // AoS rows of 3 doubles (double[N][3]) instead of 4
+  const double* q[3] = {nullptr, nullptr, nullptr};  // AoS: q[0] is double[N][4] (or [3])
   double* p[3] = {nullptr, nullptr, nullptr};
   double dt = 0, cl2 = 0, box = 0;
 };
@@ -239,5 +240,13 @@ int launch_force_mixed_f64acc(void* stream, const ListView& lv, const ForceArgs&
                               const LaunchConfig& lc);
 int launch_force_mixed_kahan(void* stream, const ListView& lv, const ForceArgs& fa,
                              const LaunchConfig& lc);
+// The same three precisions over packed double[N][3] AoS arrays
+// (force_aos3_*.hip).
+int launch_force_aos3_f64(void* stream, const ListView& lv, const ForceArgs& fa,
+                          const LaunchConfig& lc);
+int launch_force_aos3_mixed_f64acc(void* stream, const ListView& lv, const ForceArgs& fa,
+                                   const LaunchConfig& lc);
+int launch_force_aos3_mixed_kahan(void* stream, const ListView& lv, const ForceArgs& fa,
+                                  const LaunchConfig& lc);
 
 }  // namespace ljgpu

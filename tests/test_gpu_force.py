@@ -179,6 +179,82 @@ def test_f64_soa(ctx, k, rho, kind):
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
 
+def run_aos3(ctx, q, csr, box, dt=PBC_DT, steps=STEPS, lanes=4, unroll=4, tile=64, mixed=None,
+             half_mode=G.HALF_AUTO):
+    """As run_aos over packed double[N][3] q and p."""
+    ctx.set_kernel_config(lanes, 256, unroll)
+    ctx.set_tiling(tile)
+    ctx.set_half_mode(half_mode)
+    ctx.set_list(csr)
+    dq = ctx.array(np.ascontiguousarray(q[:, :3]))
+    dp = ctx.zeros((q.shape[0], 3))
+    if mixed is None:
+        ctx.force_aos3_f64(dq, dp, dt, G.CL2, box, steps)
+    else:
+        ctx.force_aos3_mixed(dq, dp, dt, G.CL2, box, steps, mixed)
+    ctx.sync()
+    out = dp.download()
+    dq.free()
+    dp.free()
+    return out
+
+
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0), (31, 0.5)])
+@pytest.mark.parametrize("kind", ["full", "half_atomic", "half_expand"])
+def test_f64_aos3(ctx, k, rho, kind):
+    """Packed double3 rows (the reference's double3 kernels): the oracle's
+    momenta at 1e-12, and the double4 call's momenta bit for bit (same
+    arithmetic), both over 100 steps (internal slot copies) and 3 steps
+    (the double3 kernels themselves)."""
+    q, box, half, full = periodic(k, rho)
+    csr = full if kind == "full" else half
+    mode = G.HALF_ATOMIC if kind == "half_atomic" else G.HALF_EXPAND
+    p3 = run_aos3(ctx, q, csr, box, half_mode=mode)
+    assert normwise(p3, oracle_momenta(k, rho)) < FP64_TOL
+    p4 = run_aos(ctx, q, csr, box, lanes=4, unroll=4, half_mode=mode)
+    if kind == "half_atomic":  # atomic scatter: the order of additions varies
+        assert normwise(p3, p4) < 1e-14
+    else:
+        np.testing.assert_array_equal(p3, p4[:, :3])
+    for lanes, unroll, tile in [(4, 4, 64), (16, 2, 0), (1, 2, 0)]:
+        a = run_aos3(ctx, q, csr, box, steps=3, lanes=lanes, unroll=unroll, tile=tile,
+                     half_mode=mode)
+        b = run_aos(ctx, q, csr, box, steps=3, lanes=lanes, unroll=unroll, tile=tile,
+                    half_mode=mode)
+        if kind == "half_atomic":  # atomic scatter: the order of additions varies
+            assert normwise(a, b) < 1e-14
+        else:
+            np.testing.assert_array_equal(a, b[:, :3])
+
+
+@pytest.mark.parametrize("accum", [G.ACCUM_F64, G.ACCUM_KAHAN_F32])
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 1.0)])
+def test_mixed_aos3(ctx, k, rho, accum):
+    q, box, _, full = periodic(k, rho)
+    F = oracle_force(k, rho)
+    m3 = run_aos3(ctx, q, full, box, dt=1.0, steps=1, mixed=accum)
+    assert normwise(m3, F) < MIXED_TOL[rho]
+    m4 = run_aos(ctx, q, full, box, dt=1.0, steps=1, lanes=4, unroll=4, mixed=accum)
+    np.testing.assert_array_equal(m3, m4[:, :3])
+    m3 = run_aos3(ctx, q, full, box, dt=0.01, steps=100, mixed=accum)
+    m4 = run_aos(ctx, q, full, box, dt=0.01, steps=100, lanes=4, unroll=4, mixed=accum)
+    np.testing.assert_array_equal(m3, m4[:, :3])
+
+
+def test_aos3_rejects_misaligned_and_short_buffers(ctx):
+    q, box, _, full = periodic(5, 1.0)
+    ctx.set_list(full)
+    n = q.shape[0]
+    dq = ctx.array(np.ascontiguousarray(q[:, :3]))
+    dp = ctx.zeros((n, 3))
+    with pytest.raises(ValueError):
+        ctx.force_aos3_f64(dq, ctx.zeros((n - 1, 3)), PBC_DT, G.CL2, box, 1)
+    rc = G.lib().ljgpu_force_aos3_f64(ctx.handle, dq.ptr + 4, dp.ptr, PBC_DT, G.CL2, box, 1)
+    assert rc == G.ERR_INVALID_ARGUMENT
+    dq.free()
+    dp.free()
+
+
 @pytest.mark.parametrize("lanes,block", [(1, 64), (4, 1024), (64, 64), (16, 512), (32, 128)])
 def test_f64_block_sizes(ctx, lanes, block):
     q, box, _, full = periodic(12, 1.0)

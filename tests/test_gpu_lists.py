@@ -117,6 +117,17 @@ def test_device_build_random_clouds(ctx):
             _same(got, host)
 
 
+@pytest.mark.parametrize("kind", [G.LIST_HALF, G.LIST_FULL])
+def test_device_build_aos3_matches_host(ctx, kind):
+    """Packed double[N][3] positions give the same list as double4 ones."""
+    q, box, half, full = periodic(12, 1.0)
+    ctx.set_half_mode(G.HALF_ATOMIC)
+    dq = ctx.array(np.ascontiguousarray(q[:, :3]))
+    ctx.build_list_aos3(dq, q.shape[0], box, kind)
+    _same(ctx.get_list(), half if kind == G.LIST_HALF else full)
+    ctx.set_half_mode(G.HALF_AUTO)
+
+
 def test_device_build_soa_matches_aos(ctx):
     q, box, half, full = periodic(12, 1.0)
     got, _ = _device_build(ctx, q, box, G.LIST_FULL, soa=True)
