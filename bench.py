@@ -268,6 +268,8 @@ def bench_md(args, G, np) -> dict:
     q, box = G.lattice_periodic(128, 0.8, 2)
     p = G.maxwell_boltzmann(q.shape[0], 1.0, 7)
     ctx = G.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     dq, dp = ctx.array(q), ctx.array(p)
     n = q.shape[0]
 
@@ -351,6 +353,9 @@ def main() -> int:
     ap.add_argument("--md-every", type=int, default=0,
                     help="md workload: rebuild the list every this many steps (0: whenever a "
                          "particle has moved skin/2 since the last build)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="force workload: run the autotuner (20-step calls) before timing, "
+                         "time its pick and report its table")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step individually instead of replaying captured graphs "
                          "(for per-dispatch profiler counters)")
@@ -399,6 +404,13 @@ def main() -> int:
     ctx.set_list(full)
     dq = ctx.array(q)
     dp = ctx.zeros(q.shape)
+
+    tune = None
+    if args.autotune:
+        t1 = time.time()
+        best, table = ctx.autotune(dq, box, reps=20)
+        tune = {"best": best, "table": table, "seconds": round(time.time() - t1, 2)}
+        log(f"autotune: {best} ({tune['seconds']} s)")
 
     f64 = lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, box, s)
     mixed = lambda s: ctx.force_aos_mixed(dq, dp, dt, G.CL2, box, s, G.ACCUM_F64)
@@ -491,6 +503,8 @@ def main() -> int:
             "rms_rel_force_err_vs_f64": err_rms,
         },
     }
+    if tune is not None:
+        result["autotune"] = tune
     if args.copy_mib > 0:
         nbytes = args.copy_mib << 20
         a = ctx.zeros(nbytes // 8)

@@ -244,11 +244,13 @@ int ljgpu_set_tile_order(ljgpu_ctx* ctx, int order);
  * *order = -1 when the kernels run untiled. Inspection only. */
 int ljgpu_get_tile_order(ljgpu_ctx* ctx, int* order, int32_t* member, int64_t capacity);
 
-/* Autotuner. Times `reps` force evaluations (after as many warm-up ones)
- * over the installed list on device AoS positions q for each candidate
- * configuration: the tiled kernel with 4, 8 or 16 lanes per particle, unroll
- * 2 or 4 and tiles of 16, 32 or 64 particles, one persistent grid, the
- * untiled wavefront-segment kernel and the thread-per-particle ELL kernel.
+/* Autotuner. For each candidate configuration, one untimed call of `reps`
+ * force evaluations (the production call shape: layouts, slot copies and
+ * graphs in place), then 5 timed calls of `reps` evaluations; the median
+ * is kept. Candidates: the tiled kernel with 4, 8 or 16 lanes per particle,
+ * unroll 2 or 4, tiles of 16, 32 or 64 particles and 1 or 2 launch chains,
+ * one persistent grid, the untiled wavefront-segment kernel and the
+ * thread-per-particle ELL kernel, over device AoS positions q.
  * The caller's momenta are not touched (a scratch buffer is used). The
  * fastest configuration is installed and returned in *best; the whole table
  * stays available through ljgpu_query_autotune. No reference counterpart (the
@@ -259,7 +261,9 @@ typedef struct ljgpu_tune_entry {
   int32_t tile_particles;   /* requested tile size (0: untiled kernels)   */
   int32_t resident_per_cu;  /* persistent grid (0: one workgroup per tile) */
   int32_t tile_used;        /* tile size the kernel ran with               */
-  float us_per_step;        /* GPU time per force evaluation               */
+  float us_per_step;        /* GPU time per force evaluation (median call)  */
+  int32_t chains;           /* concurrent launch chains (ljgpu_set_chains)  */
+  float us_min;             /* the fastest timed call                       */
 } ljgpu_tune_entry;
 
 int ljgpu_autotune(ljgpu_ctx* ctx, const double* q, double box, double cl2, int precision,

@@ -95,7 +95,8 @@ class TuneEntry(C.Structure):
     """ljgpu_tune_entry (include/ljgpu.h)."""
     _fields_ = [("lanes_per_particle", C.c_int32), ("unroll", C.c_int32),
                 ("tile_particles", C.c_int32), ("resident_per_cu", C.c_int32),
-                ("tile_used", C.c_int32), ("us_per_step", C.c_float)]
+                ("tile_used", C.c_int32), ("us_per_step", C.c_float), ("chains", C.c_int32),
+                ("us_min", C.c_float)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

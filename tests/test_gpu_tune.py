@@ -22,9 +22,11 @@ def test_autotune_installs_the_fastest_configuration(ctx):
     ctx.set_half_mode(G.HALF_AUTO)
     ctx.set_list(half)
     dq = ctx.array(q)
-    best, table = ctx.autotune(dq, box, reps=10)
-    assert len(table) == 22
+    best, table = ctx.autotune(dq, box, reps=16)
+    assert len(table) == 40
     assert best["us_per_step"] == min(e["us_per_step"] for e in table)
+    assert all(0 < e["us_min"] <= e["us_per_step"] for e in table)
+    assert {e["chains"] for e in table} == {1, 2}
     info = ctx.list_info()
     assert (info.lanes_per_particle, info.unroll) == (best["lanes_per_particle"], best["unroll"])
     assert info.tile_particles == best["tile_used"]
@@ -43,7 +45,7 @@ def test_autotune_mixed_and_replicas(ctx):
     best, table = ctx.autotune(dq, 0.0, precision=G.PREC_MIXED, reps=5)
     # Replicas run on the tiled kernel only: the untiled and ELL candidates
     # are skipped.
-    assert len(table) == 19
+    assert len(table) == 37
     assert all(e["tile_used"] > 0 for e in table)
     assert best["tile_used"] > 0
     ctx.set_replicas([], [])
