@@ -377,7 +377,9 @@ int ljgpu_md_run(ljgpu_ctx* ctx, double* q, double* p, int32_t particle_number, 
 /* Potential energy sum of 4 (r^-12 - r^-6) over the installed list's pairs
  * with r2 <= cl2 (the force kernels' cutoff rule, no shift), kinetic energy
  * sum of |p|^2 / 2, device AoS q and p, and (optional) the number of pairs
- * inside the cutoff. Synchronous. */
+ * inside the cutoff. Over an installed LDS tiling (the force kernels' layout,
+ * used as it stands) the potential runs on the tiles, otherwise over the
+ * list's rows. Synchronous. */
 int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
                  double* potential, double* kinetic, int64_t* pairs_in_cutoff);
 
@@ -388,8 +390,8 @@ int ljgpu_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, d
  * partials are added in the same order). Its partial sums live in a static
  * per-GPU array of 3 * 1024 + 3 doubles (24.6 KB, held by every GPU the
  * library runs on until the process ends) instead of context memory. The
- * grid is capped at 1,024 workgroups that stride over the rows, so no
- * particle count or list length needs more scratch than that. All calls on
+ * grids are capped at 1,024 workgroups that stride over the rows or tiles,
+ * so no particle count or list length needs more scratch than that. All calls on
  * one GPU share the array: calls that overlap on one GPU (from several
  * threads, or contexts on different streams) race and must be serialised by
  * the caller. The list must already be installed (installing one

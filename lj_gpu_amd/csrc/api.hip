@@ -344,31 +344,35 @@ int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* 
     lv->ell_width = ctx->ell_width;
   } else {
     lv->list = ctx->eval.list.as<int32_t>();
-    if (ctx->tiles.blocks > 0 && !ctx->eval_half) {
-      lv->tile_blocks = ctx->tiles.blocks;
-      lv->tile_particles = ctx->tiles.tile_particles;
-      lv->tile_lds_bytes = ctx->tiles.lds_bytes;
-      lv->tile_max_stage = ctx->tiles.max_stage;
-      lv->tile_desc = ctx->tiles.desc.as<int32_t>();
-      lv->tile_stage = ctx->tiles.stage.as<int32_t>();
-      lv->tile_local = ctx->tiles.local.as<uint16_t>();
-      if (ctx->tiles.sliced_lanes > 0) {
-        lv->tile_sliced = ctx->tiles.sliced.as<uint16_t>();
-        lv->tile_wave_off = ctx->tiles.wave_off.as<int32_t>();
-        lv->tile_waves_per_block = ctx->tiles.waves_per_block;
-        lv->tile_sliced_lanes = ctx->tiles.sliced_lanes;
-        lv->tile_sliced_unroll = ctx->tiles.sliced_unroll;
-      }
-      if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL) {
-        lv->tile_member = ctx->tiles.member.as<int32_t>();
-        lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
-        if (ctx->tiles.staged > 0) lv->tile_stage_slot = ctx->tiles.stage_slot.as<int32_t>();
-      } else {
-        lv->tile_slot_ptr = ctx->eval.ptr.as<int32_t>();  // slot = particle
-      }
-    }
+    if (ctx->tiles.blocks > 0 && !ctx->eval_half) tile_fields(ctx, lv);
   }
   return LJGPU_OK;
+}
+
+// The installed tiling's fields of a ListView (as they stand: nothing is
+// derived or rebuilt).
+void tile_fields(const ljgpu_ctx* ctx, ListView* lv) {
+  lv->tile_blocks = ctx->tiles.blocks;
+  lv->tile_particles = ctx->tiles.tile_particles;
+  lv->tile_lds_bytes = ctx->tiles.lds_bytes;
+  lv->tile_max_stage = ctx->tiles.max_stage;
+  lv->tile_desc = ctx->tiles.desc.as<int32_t>();
+  lv->tile_stage = ctx->tiles.stage.as<int32_t>();
+  lv->tile_local = ctx->tiles.local.as<uint16_t>();
+  if (ctx->tiles.sliced_lanes > 0) {
+    lv->tile_sliced = ctx->tiles.sliced.as<uint16_t>();
+    lv->tile_wave_off = ctx->tiles.wave_off.as<int32_t>();
+    lv->tile_waves_per_block = ctx->tiles.waves_per_block;
+    lv->tile_sliced_lanes = ctx->tiles.sliced_lanes;
+    lv->tile_sliced_unroll = ctx->tiles.sliced_unroll;
+  }
+  if (ctx->tiles.order == LJGPU_TILE_ORDER_SPATIAL) {
+    lv->tile_member = ctx->tiles.member.as<int32_t>();
+    lv->tile_slot_ptr = ctx->tiles.slot_ptr.as<int32_t>();
+    if (ctx->tiles.staged > 0) lv->tile_stage_slot = ctx->tiles.stage_slot.as<int32_t>();
+  } else {
+    lv->tile_slot_ptr = ctx->eval.ptr.as<int32_t>();  // slot = particle
+  }
 }
 
 int launch_one_on(hipStream_t stream, const ListView& lv, const LaunchConfig& lc,

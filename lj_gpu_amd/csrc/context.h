@@ -195,6 +195,8 @@ void drop_graph(ljgpu_ctx* ctx);
 // Kernel arguments of one force evaluation over the installed list (derives
 // missing layouts first, from the positions of `fa`).
 int force_view(ljgpu_ctx* ctx, const ForceArgs& fa, ListView* lv, LaunchConfig* lc);
+// The installed tiling's fields of a ListView, as they stand.
+void tile_fields(const ljgpu_ctx* ctx, ListView* lv);
 // Record the launches `body` enqueues on the context stream into an
 // executable graph.
 int capture(ljgpu_ctx* ctx, const std::function<int()>& body, hipGraphExec_t* out);

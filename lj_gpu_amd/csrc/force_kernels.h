@@ -38,6 +38,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lj_device.h"
@@ -49,6 +50,10 @@ namespace detail {
 constexpr int kF64 = 0;
 constexpr int kMixedF64 = 1;
 constexpr int kMixedKahan = 2;
+// Not a force policy: the tiled potential-energy kernel (k_tile_energy)
+// runs the tile code with this policy, accumulating per pair
+// s^3 (s^3 - 1) with s = 1/r2 and the pair count instead of the force.
+constexpr int kEnergy = 3;
 
 // Per-lane accumulator of the i-particle's force for each precision policy.
 template <int PREC>
@@ -75,6 +80,20 @@ struct Accum<kMixedKahan> {
   __device__ __forceinline__ double vx() const { return x.value(); }
   __device__ __forceinline__ double vy() const { return y.value(); }
   __device__ __forceinline__ double vz() const { return z.value(); }
+};
+
+template <>
+struct Accum<kEnergy> {
+  double u = 0.0, c = 0.0;
+  // s = 1/r2 (refined reciprocal); nothing unless the pair is inside.
+  __device__ __forceinline__ void add_pair(double s, bool inside) {
+    const double s3 = s * s * s;
+    u += inside ? s3 * (s3 - 1.0) : 0.0;
+    c += inside ? 1.0 : 0.0;
+  }
+  __device__ __forceinline__ double vx() const { return u; }
+  __device__ __forceinline__ double vy() const { return c; }
+  __device__ __forceinline__ double vz() const { return 0.0; }
 };
 
 // One list entry once q_j is in registers: minimum image, cutoff, force
@@ -340,6 +359,8 @@ __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restr
     if constexpr (PREC == kF64) {
       acc.add(lj_df_unit_f64(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]), 0.0f, dx[u],
               dy[u], dz[u]);
+    } else if constexpr (PREC == kEnergy) {
+      acc.add_pair(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]);
     } else {
       float dff = lj_df_mixed(r2[u], pp);
       if constexpr (LIMIT) dff = valid[u] ? dff : 0.0f;
@@ -424,7 +445,7 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
                                           const int32_t* __restrict__ wave_off, int wpb, int b,
                                           const PairParams& gpp, double greach,
                                           const double2* __restrict__ rep_box, char* lb,
-                                          int* flag) {
+                                          int* flag, double* eacc) {
   const int4 d0 = desc[2 * b];
   const int4 d1 = desc[2 * b + 1];
   const int i0 = d0.x, ni = d0.y, soff = d0.z, ns = d0.w;
@@ -531,15 +552,25 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
         tile_row_stream<TPI, U, PREC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc);
     }
   }
-  double fx = segment_sum<TPI>(acc.vx());
-  double fy = segment_sum<TPI>(acc.vy());
-  double fz = segment_sum<TPI>(acc.vz());
-  if constexpr (PREC == kF64) {
-    fx *= pp.c24;
-    fy *= pp.c24;
-    fz *= pp.c24;
+  if constexpr (PREC == kEnergy) {
+    // Row sums of the energy terms and pair counts into the thread's totals.
+    const double ru = segment_sum<TPI>(acc.vx());
+    const double rc = segment_sum<TPI>(acc.vy());
+    if (active && sub == 0) {
+      eacc[0] += ru;
+      eacc[1] += rc;
+    }
+  } else {
+    double fx = segment_sum<TPI>(acc.vx());
+    double fy = segment_sum<TPI>(acc.vy());
+    double fz = segment_sum<TPI>(acc.vz());
+    if constexpr (PREC == kF64) {
+      fx *= pp.c24;
+      fy *= pp.c24;
+      fz *= pp.c24;
+    }
+    if (active && sub == 0) mom.add(i, fx, fy, fz);
   }
-  if (active && sub == 0) mom.add(i, fx, fy, fz);
 }
 
 // At most 96 VGPRs (five waves per SIMD): holding every entry's distance of a
@@ -568,8 +599,57 @@ k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict_
   for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
     tile_body<TPI, U, PREC, P, M, PBC, OFFS_LDS, SL>(pos, mom, nop, member, slot_ptr, desc, stage,
                                                      local, sliced, wave_off, wpb, first + b, pp,
-                                                     reach, rep_box, lb, flag);
+                                                     reach, rep_box, lb, flag, nullptr);
     __syncthreads();  // the next tile overwrites this tile's LDS
+  }
+}
+
+// Potential energy over the tiles (the force kernel's staging, offsets and
+// pair arithmetic, with the energy policy): each workgroup walks its tiles
+// as k_tile does and writes weight * 4 * sum s^3 (s^3 - 1) and weight * the
+// pair count into part_u / part_c[blockIdx.x] (fixed-order sums, so the
+// result does not depend on scheduling). Row offsets from `local`.
+template <int TPI, int U, bool PBC>
+__global__ void __launch_bounds__(1024)
+k_tile_energy(PosAos pos, const int32_t* __restrict__ member,
+              const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
+              const int32_t* __restrict__ stage, const uint16_t* __restrict__ local,
+              int32_t nblocks, PairParams pp, double reach, double weight,
+              double* __restrict__ part_u, double* __restrict__ part_c) {
+  extern __shared__ double lds[];
+  int* flag = reinterpret_cast<int*>(lds);
+  char* lb = reinterpret_cast<char*>(lds) + kTileLdsHeader;
+  if (threadIdx.x == 0) *flag = -1;
+  __syncthreads();
+  double eacc[2] = {0.0, 0.0};
+  const int xcd = static_cast<int>(blockIdx.x) % kXcds;
+  const int per = (nblocks + kXcds - 1) / kXcds;
+  const int stride = static_cast<int>(gridDim.x) / kXcds;
+  const int t_end = min(nblocks, (xcd + 1) * per);
+  for (int b = xcd * per + static_cast<int>(blockIdx.x) / kXcds; b < t_end; b += stride) {
+    tile_body<TPI, U, kEnergy, PosAos, MomAos, PBC, false, false>(
+        pos, MomAos{nullptr}, nullptr, member, slot_ptr, desc, stage, local, nullptr, nullptr, 0,
+        b, pp, reach, nullptr, lb, flag, eacc);
+    __syncthreads();
+  }
+  // Wavefront sums, then the wavefronts' sums in order by thread 0.
+  const double wu = segment_sum<64>(eacc[0]);
+  const double wc = segment_sum<64>(eacc[1]);
+  double* red = lds + 2;  // past the header word, free after the last barrier
+  const int w = static_cast<int>(threadIdx.x) / 64;
+  if ((threadIdx.x & 63) == 0) {
+    red[2 * w] = wu;
+    red[2 * w + 1] = wc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double u = 0.0, c = 0.0;
+    for (int k = 0; k < static_cast<int>(blockDim.x) / 64; k++) {
+      u += red[2 * k];
+      c += red[2 * k + 1];
+    }
+    part_u[blockIdx.x] = 4.0 * weight * u;
+    part_c[blockIdx.x] = weight * c;
   }
 }
 
@@ -584,6 +664,39 @@ inline int launch_status(const char* what) {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? LJGPU_OK
                          : set_error(LJGPU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Launch k_tile_energy over the installed tiling with `grid` workgroups (a
+// multiple of 8, at most the partial arrays' length). Four lanes per
+// particle and four entries per batch, whatever the force configuration
+// (the `local` offsets serve every configuration).
+inline int run_energy_tiles(hipStream_t s, const double* q, const ListView& lv, double cl2,
+                            double box, double weight, int grid, double* part_u,
+                            double* part_c) {
+  constexpr int kLanes = 4;
+  const int threads = (lv.tile_particles * kLanes + 63) / 64 * 64;
+  if (threads > 1024 || grid % kXcds != 0 || grid <= 0)
+    return set_error(LJGPU_ERR_INVALID_ARGUMENT, "tiled energy: launch shape");
+  PairParams pp{};
+  pp.cl2 = cl2;
+  pp.box = box > 0.0 ? box : 0.0;
+  pp.inv_box = box > 0.0 ? 1.0 / box : 0.0;
+  const double reach = 0.5 * pp.box - sqrt(cl2);
+  // The header, the staged positions, and room for the final reduction.
+  const size_t lds_bytes =
+      kTileLdsHeader + std::max<size_t>(static_cast<size_t>(lv.tile_max_stage) * 24, 16 * 16);
+  PosAos pos{q};
+  if (box > 0.0)
+    hipLaunchKernelGGL((k_tile_energy<kLanes, 4, true>), dim3(grid), dim3(threads), lds_bytes, s,
+                       pos, lv.tile_member, lv.tile_slot_ptr,
+                       reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
+                       lv.tile_blocks, pp, reach, weight, part_u, part_c);
+  else
+    hipLaunchKernelGGL((k_tile_energy<kLanes, 4, false>), dim3(grid), dim3(threads), lds_bytes, s,
+                       pos, lv.tile_member, lv.tile_slot_ptr,
+                       reinterpret_cast<const int4*>(lv.tile_desc), lv.tile_stage, lv.tile_local,
+                       lv.tile_blocks, pp, reach, weight, part_u, part_c);
+  return launch_status("tiled energy kernel launch");
 }
 
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool HALF, bool CLAMP>

@@ -240,6 +240,13 @@ int launch_force_mixed_f64acc(void* stream, const ListView& lv, const ForceArgs&
                               const LaunchConfig& lc);
 int launch_force_mixed_kahan(void* stream, const ListView& lv, const ForceArgs& fa,
                              const LaunchConfig& lc);
+// Potential energy over an installed tiling (AoS double4 q): weight * 4 *
+// sum over pairs inside cl2 of s^3 (s^3 - 1), s = 1/r2, and weight * the
+// pair count, one partial per workgroup of `grid` (a multiple of 8) in
+// part_u / part_c (force_f64.hip).
+int launch_energy_tiles(void* stream, const double* q, const ListView& lv, double cl2,
+                        double box, double weight, int grid, double* part_u, double* part_c);
+
 // The same three precisions over packed double[N][3] AoS arrays
 // (force_aos3_*.hip).
 int launch_force_aos3_f64(void* stream, const ListView& lv, const ForceArgs& fa,

@@ -204,37 +204,50 @@ int grid_for(int64_t n) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65535)));
 }
 
-// Sums of the energy partials in block order b = 0, 1, ..., grid - 1 from
-// 0.0 (the order of energy()'s host loop, so both give the same doubles):
-// out = (U, K, pairs).
-__global__ void k_energy_total(int grid, int with_q, int with_p,
+// Sums of the energy partials in block order b = 0, 1, ... from 0.0 (the
+// order of energy()'s host loop, so both give the same doubles): out =
+// (U, K, pairs); grid_q partials of U and pairs, grid_p of K.
+__global__ void k_energy_total(int grid_q, int grid_p, int with_q, int with_p,
                                const double* __restrict__ part, double* __restrict__ out) {
   if (threadIdx.x != 0) return;
   double u = 0.0, k = 0.0, c = 0.0;
-  for (int b = 0; b < grid; b++) {
-    if (with_q) u += part[b];
-    if (with_p) k += part[kEnergyBlocks + b];
-    if (with_q) c += part[2 * kEnergyBlocks + b];
-  }
+  if (with_q)
+    for (int b = 0; b < grid_q; b++) u += part[b];
+  if (with_p)
+    for (int b = 0; b < grid_p; b++) k += part[kEnergyBlocks + b];
+  if (with_q)
+    for (int b = 0; b < grid_q; b++) c += part[2 * kEnergyBlocks + b];
   out[0] = u;
   out[1] = k;
   out[2] = c;
 }
 
 // Launch the energy kernels of the current state into `part` (3 *
-// kEnergyBlocks doubles: potential, kinetic and pair-count partials, one
-// per workgroup of the returned grid).
+// kEnergyBlocks doubles: potential, kinetic and pair-count partials). Over
+// an installed tiling (the force kernels' layout, used as it stands) the
+// potential runs on the tiles (k_tile_energy), otherwise over the CSR rows
+// (k_potential). *grid_q / *grid_p: the partials of U and pairs / of K.
 int launch_energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double cl2,
-                  double* part, int* grid_out) {
+                  double* part, int* grid_q, int* grid_p) {
   const int grid = std::min(kEnergyBlocks, grid_for(ctx->n));
-  *grid_out = grid;
+  *grid_q = *grid_p = grid;
   if (q) {
     const double weight = ctx->eval_half ? 1.0 : 0.5;
-    k_potential<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, ctx->eval.nop.as<int32_t>(),
-                                                  ctx->eval.ptr.as<int32_t>(),
-                                                  ctx->eval.list.as<int32_t>(), q, cl2, box,
-                                                  box > 0.0 ? 1.0 / box : 0.0, weight, part,
-                                                  part + 2 * kEnergyBlocks);
+    const bool tiled = ctx->tiles.blocks > 0 && !ctx->eval_half && ctx->rep_off.empty() &&
+                       ctx->tiles.tile_particles * 4 <= 1024;
+    if (tiled) {
+      ListView lv;
+      tile_fields(ctx, &lv);
+      *grid_q = std::min(kEnergyBlocks, (ctx->tiles.blocks + 7) / 8 * 8);
+      TRY(launch_energy_tiles(ctx->stream, q, lv, cl2, box, weight, *grid_q, part,
+                              part + 2 * kEnergyBlocks));
+    } else {
+      k_potential<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, ctx->eval.nop.as<int32_t>(),
+                                                    ctx->eval.ptr.as<int32_t>(),
+                                                    ctx->eval.list.as<int32_t>(), q, cl2, box,
+                                                    box > 0.0 ? 1.0 / box : 0.0, weight, part,
+                                                    part + 2 * kEnergyBlocks);
+    }
   }
   if (p) k_kinetic<<<grid, kBlock, 0, ctx->stream>>>(ctx->n, p, part + kEnergyBlocks);
   HIP_TRY(hipGetLastError(), "energy kernels");
@@ -251,17 +264,18 @@ int energy(ljgpu_ctx* ctx, const double* q, const double* p, double box, double 
   if (ctx->n == 0) return LJGPU_OK;
   TRY(dev_reserve(ctx->md_red, sizeof(double) * 3 * kEnergyBlocks + 64, ctx->stream));
   double* part = ctx->md_red.as<double>();
-  int grid = 0;
-  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid));
+  int grid_q = 0, grid_p = 0;
+  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid_q, &grid_p));
   std::vector<double> h(3 * kEnergyBlocks);
   HIP_TRY(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
                          ctx->stream), "hipMemcpyAsync");
   HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  for (int b = 0; b < grid; b++) {
-    if (q) *u += h[b];
-    if (p) *k += h[kEnergyBlocks + b];
-    if (q) *pairs += h[2 * kEnergyBlocks + b];
-  }
+  if (q)
+    for (int b = 0; b < grid_q; b++) *u += h[b];
+  if (p)
+    for (int b = 0; b < grid_p; b++) *k += h[kEnergyBlocks + b];
+  if (q)
+    for (int b = 0; b < grid_q; b++) *pairs += h[2 * kEnergyBlocks + b];
   return LJGPU_OK;
 }
 
@@ -308,9 +322,10 @@ int ljgpu_energy_noalloc(ljgpu_ctx* ctx, const double* q, const double* p, doubl
   // workgroup of the grid), so nothing in the scratch needs clearing.
   double* part = ctx->noalloc_scratch;
   double* total = part + 3 * kEnergyBlocks;
-  int grid = 0;
-  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid));
-  k_energy_total<<<1, 64, 0, ctx->stream>>>(grid, q != nullptr, p != nullptr, part, total);
+  int grid_q = 0, grid_p = 0;
+  TRY(launch_energy(ctx, q, p, box, cl2, part, &grid_q, &grid_p));
+  k_energy_total<<<1, 64, 0, ctx->stream>>>(grid_q, grid_p, q != nullptr, p != nullptr, part,
+                                            total);
   HIP_TRY(hipGetLastError(), "k_energy_total");
   double h[3] = {0.0, 0.0, 0.0};
   HIP_TRY(hipMemcpyAsync(h, total, sizeof(h), hipMemcpyDeviceToHost, ctx->stream),

@@ -86,6 +86,33 @@ def test_energy_matches_all_pairs_sum(ctx, kind):
     assert k == pytest.approx(0.5 * (p[:, :3] ** 2).sum(), rel=1e-13)
 
 
+@pytest.mark.parametrize("k,rho", [(6, 0.8), (12, 1.0), (31, 0.5)])
+def test_tiled_energy_matches_row_energy(ctx, k, rho):
+    """The potential over the LDS tiles (k_tile_energy) against the same
+    sum over the CSR rows (k_potential, tiling off), and the pair counts
+    exactly."""
+    q, box = G.lattice_periodic(k, rho, 2)
+    p = G.maxwell_boltzmann(q.shape[0], 1.0, 7)
+    full = G.makepair(q, box, G.LIST_FULL)
+    dq, dp = ctx.array(q), ctx.array(p)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(0)
+    ctx.set_list(full)
+    assert ctx.list_info().tile_particles == 0
+    rows = ctx.energy_pairs(dq, dp, box)
+    ctx.set_tiling(64)
+    ctx.set_list(full)
+    assert ctx.list_info().tile_particles > 0
+    tiles = ctx.energy_pairs(dq, dp, box)
+    ctx.force_aos_f64(dq, ctx.zeros(q.shape), 0.01, G.CL2, box, 1)  # spatial tiles
+    spatial = ctx.energy_pairs(dq, dp, box)
+    for got in (tiles, spatial):
+        assert got[0] == pytest.approx(rows[0], rel=1e-13)
+        assert got[1] == rows[1]
+        assert got[2] == rows[2]
+
+
 def test_force_is_minus_energy_gradient(ctx):
     q, p, box = liquid(6)
     full = G.makepair(q, box, G.LIST_FULL)
