@@ -1180,21 +1180,36 @@ __global__ void k_wave_slices(const int32_t* __restrict__ desc, int32_t nblocks,
   len[id] = batches * 64 * unroll;
 }
 
-// Every entry offset to its slice position (one workgroup per block).
-__global__ __launch_bounds__(kBlock) void k_slice_scatter(
+// The slices of one block per workgroup, written in order: slice position
+// p of wavefront w (batch j = p / (64 unroll), slot u, lane l) holds entry
+// k = j * lanes * unroll + u * lanes + l % lanes of the block's row
+// w * (64 / lanes) + l / lanes, or 0 past the row's end (padding). The
+// writes are contiguous; the reads gather short runs of `local` (the
+// block's rows, a few KB, served by the cache). Row starts come from LDS.
+__global__ __launch_bounds__(kBlock) void k_slice_gather(
     const int32_t* __restrict__ desc, int32_t nblocks, int32_t wpb,
     const int32_t* __restrict__ slot_ptr, const uint16_t* __restrict__ local, int32_t lanes,
     int32_t unroll, const int32_t* __restrict__ wave_off, uint16_t* __restrict__ sliced) {
-  const int32_t rpw = 64 / lanes, kb = lanes * unroll;
+  __shared__ int32_t rs[kTileRowsMax + 1];
+  const int32_t rpw = 64 / lanes, span = 64 * unroll;
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const int32_t i0 = desc[8 * static_cast<int64_t>(b)], ni = desc[8 * static_cast<int64_t>(b) + 1];
-    for (int32_t r = 0; r < ni; r++) {
-      const int32_t a = slot_ptr[i0 + r], np = slot_ptr[i0 + r + 1] - a;
-      const int64_t base = wave_off[static_cast<int64_t>(b) * wpb + r / rpw] + (r % rpw) * lanes;
-      for (int32_t k = threadIdx.x; k < np; k += kBlock)
-        sliced[base + static_cast<int64_t>(k / kb) * 64 * unroll + ((k / lanes) % unroll) * 64 +
-               k % lanes] = local[a + k];
+    for (int32_t r = threadIdx.x; r <= ni; r += kBlock) rs[r] = slot_ptr[i0 + r];
+    __syncthreads();
+    const int32_t* wo = wave_off + static_cast<int64_t>(b) * wpb;
+    const int32_t p0 = wo[0], p1 = wo[wpb];  // the block's slices, back to back
+    for (int32_t p = p0 + static_cast<int32_t>(threadIdx.x); p < p1; p += kBlock) {
+      int32_t w = 0;
+      while (w + 1 < wpb && wo[w + 1] <= p) w++;
+      const int32_t off = p - wo[w];
+      const int32_t j = off / span, u = (off % span) >> 6, l = off & 63;
+      const int32_t row = w * rpw + l / lanes;
+      const int32_t k = j * lanes * unroll + u * lanes + l % lanes;
+      uint16_t v = 0;
+      if (row < ni && k < rs[row + 1] - rs[row]) v = local[rs[row] + k];
+      sliced[p] = v;
     }
+    __syncthreads();
   }
 }
 
@@ -1943,11 +1958,11 @@ int dev_slice_tiles(hipStream_t s, const int32_t* slot_ptr, int32_t lanes, int32
   const int64_t pad = 2 * 64 * static_cast<int64_t>(unroll) + 64;
   if (total + pad > INT32_MAX) return LJGPU_OK;
   LCHECK(dev_reserve(t.sliced, sizeof(uint16_t) * (total + pad), s));
-  LCHECK(fill<uint16_t>(s, t.sliced.as<uint16_t>(), total + pad, 0));
-  k_slice_scatter<<<std::min(t.blocks, 65535), kBlock, 0, s>>>(
+  LCHECK(fill<uint16_t>(s, t.sliced.as<uint16_t>() + total, pad, 0));
+  k_slice_gather<<<std::min(t.blocks, 65535), kBlock, 0, s>>>(
       t.desc.as<int32_t>(), t.blocks, wpb, slot_ptr, t.local.as<uint16_t>(), lanes, unroll,
       t.wave_off.as<int32_t>(), t.sliced.as<uint16_t>());
-  LTRY(hipGetLastError(), "slice scatter kernel");
+  LTRY(hipGetLastError(), "slice gather kernel");
   t.sliced_lanes = lanes;
   t.sliced_unroll = unroll;
   t.waves_per_block = wpb;
