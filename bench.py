@@ -166,6 +166,8 @@ def bench_large(args, G, np) -> dict:
     dt, k = 0.01, 64
     q, box = G.lattice_periodic(k, 1.0, 2)
     ctx = G.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     dq = ctx.array(q)
     ctx.build_list(dq, q.shape[0], box, G.LIST_FULL)
     info = ctx.list_info()
@@ -222,6 +224,8 @@ def bench_replicas(args, G, np) -> dict:
     dt = 0.01
     q, off, boxes = G.replica_workload(256, seed=4)
     ctx = G.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     ctx.set_replicas(off, boxes)
     dq = ctx.array(q)
     ctx.build_list(dq, q.shape[0], 0.0, G.LIST_FULL)
