@@ -24,9 +24,12 @@
 //                           the compiled cpu_ref force_pair/force_sorted;
 //                           two-body closed forms, Newton's third law,
 //                           half-vs-full-list agreement
-//   orc_random_shfl         parity unpinned (cuda/force_cuda.cu cannot be
-//                           compiled here); only permutation invariants and
-//                           force invariance under row shuffles are checked
+//   orc_random_shfl         an independent restatement of the permutation
+//                           (tests/test_random_shfl.py): numpy's legacy
+//                           MT19937 seeded with 10 is std::mt19937(10), and
+//                           libstdc++'s std::shuffle over its
+//                           uniform_int_distribution is restated in Python;
+//                           every row, one stream across rows, must match
 #include <cmath>
 #include <cstdint>
 #include <cstring>
