@@ -524,6 +524,18 @@ __global__ void k_slot_copy_aos(int32_t n, const int32_t* __restrict__ member,
   }
 }
 
+// q and p gathered into slot order in one pass (out[s] = in[member[s]]).
+__global__ void k_slot_gather_qp(int32_t n, const int32_t* __restrict__ member,
+                                 const double4* __restrict__ q, const double4* __restrict__ p,
+                                 double4* __restrict__ oq, double4* __restrict__ op) {
+  for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t i = member[t];
+    oq[t] = q[i];
+    op[t] = p[i];
+  }
+}
+
 template <bool GATHER>
 __global__ void k_slot_copy_soa(int32_t n, const int32_t* __restrict__ member, const double* x,
                                 const double* y, const double* z, double* ox, double* oy,
@@ -579,10 +591,9 @@ int slot_gather(ljgpu_ctx* ctx, const ForceArgs& fa, ForceArgs* out) {
     out->q[0] = q;
     out->p[0] = p;
   } else if (!fa.soa) {
-    k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
-        n, mem, reinterpret_cast<const double4*>(fa.q[0]), reinterpret_cast<double4*>(q));
-    k_slot_copy_aos<true><<<slot_grid(n), 256, 0, ctx->stream>>>(
-        n, mem, reinterpret_cast<const double4*>(fa.p[0]), reinterpret_cast<double4*>(p));
+    k_slot_gather_qp<<<slot_grid(n), 256, 0, ctx->stream>>>(
+        n, mem, reinterpret_cast<const double4*>(fa.q[0]), reinterpret_cast<const double4*>(fa.p[0]),
+        reinterpret_cast<double4*>(q), reinterpret_cast<double4*>(p));
     out->q[0] = q;
     out->p[0] = p;
   } else {

@@ -12,7 +12,8 @@
 - FCC 4*128^3 = 8,388,608 particles at density 0.8: the step-0 force of a
   sample of particles against a brute-force sum over all particles, and
   1,000 velocity-Verlet steps checked through properties that hold at any
-  size (total momentum, the shifted energy, stale-list reporting).
+  size (total momentum, the shifted energy, stale-list reporting) and the
+  force at the final positions against a brute-force sum.
 """
 from __future__ import annotations
 
@@ -262,6 +263,19 @@ def test_md_full_size_properties(ctx, md_system, policy):
           f"{st.max_displacement:.3f} E0 {e0:.6f} rel drift {rel:.3e} sum p {p0} -> {p1}",
           flush=True)
     assert st.steps == 1000 and st.samples == samples.shape[0] == 11
+    # The force at the final positions (a fresh device list) against the
+    # all-particle brute force, for a sample of particles.
+    dq1 = ctx.array(q1)
+    ctx.build_list(dq1, q1.shape[0], box, G.LIST_FULL)
+    dF = ctx.zeros(q1.shape)
+    ctx.force_aos_f64(dq1, dF, 1.0, G.CL2, box, 1)
+    ctx.sync()
+    F1 = dF.download()
+    dq1.free()
+    dF.free()
+    sample = np.random.default_rng(23).choice(q1.shape[0], 12, replace=False)
+    ref = np.array([_brute_force(q1, box, int(i)) for i in sample])
+    assert normwise(F1[sample, :3], ref) < FP64_TOL
     # Each pair enters both rows of the full list, so the total momentum
     # moves only by rounding: of each particle's sum, and of the pair
     # differences, which the tiles take in a frame shifted by whole boxes
