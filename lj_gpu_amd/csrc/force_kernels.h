@@ -459,7 +459,7 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
     const double2 bb = rep_box[d1.z];
     pp.box = bb.x;
     pp.inv_box = bb.y;
-    reach = 0.5 * bb.x - sqrt(pp.cl2);
+    reach = 0.5 * bb.x - pp.rc;
   }
   double* lds = reinterpret_cast<double*>(lb);
   const int off_area = (ns * 24 + 15) & ~15;
@@ -680,6 +680,7 @@ inline int run_energy_tiles(hipStream_t s, const double* q, const ListView& lv, 
     return set_error(LJGPU_ERR_INVALID_ARGUMENT, "tiled energy: launch shape");
   PairParams pp{};
   pp.cl2 = cl2;
+  pp.rc = sqrt(cl2);
   pp.box = box > 0.0 ? box : 0.0;
   pp.inv_box = box > 0.0 ? 1.0 / box : 0.0;
   const double reach = 0.5 * pp.box - sqrt(cl2);
@@ -862,6 +863,7 @@ inline int pair_params(const ForceArgs& fa, const LaunchConfig& lc, PairParams* 
   pp->inv_box = fa.box > 0.0 ? 1.0 / fa.box : 0.0;
   pp->c24f = static_cast<float>(pp->c24);
   pp->c48f = static_cast<float>(pp->c48);
+  pp->rc = sqrt(fa.cl2);
   return LJGPU_OK;
 }
 

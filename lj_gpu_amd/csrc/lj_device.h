@@ -118,9 +118,10 @@ struct MomSoa {
 
 // Pair constants: c24 = 24*dt and c48 = 48*dt fold the time step into the
 // force scalar; cl2 is the squared cutoff (pairs with r2 > cl2 contribute
-// nothing, r2 == cl2 is kept, as in the reference's `if (r2 > CL2)`).
+// nothing, r2 == cl2 is kept, as in the reference's `if (r2 > CL2)`), rc its
+// square root (computed on the host, so no kernel spends VALU on it).
 struct PairParams {
-  double c24, c48, cl2, box, inv_box;
+  double c24, c48, cl2, box, inv_box, rc;
   float c24f, c48f;
 };
 
