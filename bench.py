@@ -316,6 +316,28 @@ def bench_md(args, G, np) -> dict:
     }
 
 
+def bench_reference_mode(args, G, np) -> dict:
+    """The reference's own problem (cpu_ref/force_aos.cpp:43-51 init(): open
+    box L = 50, density 0.5, U(0, 0.1) jitter, 62,500 particles; dt 0.001),
+    FP64 over its full list, timed like the headline: 100-step calls, median
+    of the repeats. The step the reference's measure() loop times."""
+    dt = 0.001
+    q = G.lattice_reference(0.5, 50.0)
+    full = G.makepair(q, 0.0, G.LIST_FULL)
+    ctx = G.Context(0)
+    ctx.set_list(full)
+    dq, dp = ctx.array(q), ctx.zeros(q.shape)
+    t = time_leg(ctx, lambda s: ctx.force_aos_f64(dq, dp, dt, G.CL2, 0.0, s), 100, 100,
+                 args.repeats, 50.0)
+    ctx.close()
+    pairs = full.n_entries // 2
+    return {"workload": f"reference problem: open box L=50, density 0.5, N={full.n}, full list "
+                        f"({full.n_entries} entries, {pairs} unique pairs), dt {dt}, AoS double4",
+            "ms_per_step": round(t["ms_per_step"], 7),
+            "value": round(pairs / t["ms_per_step"] / 1e6, 3), "unit": "Gpair-interactions/s",
+            "steps_per_call": 100, "repeats_ms_per_step": t["repeats_ms_per_step"]}
+
+
 def roofline(bytes_per_launch: int, seconds: float, traffic=None) -> dict:
     achieved = bytes_per_launch / seconds / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -360,6 +382,9 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="force workload: run the autotuner (20-step calls) before timing, "
                          "time its pick and report its table")
+    ap.add_argument("--no-reference-mode", action="store_true",
+                    help="force workload: skip the timing of the reference's own open-box "
+                         "problem")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every step individually instead of replaying captured graphs "
                          "(for per-dispatch profiler counters)")
@@ -509,6 +534,12 @@ def main() -> int:
     }
     if tune is not None:
         result["autotune"] = tune
+    # The reference's own problem, in the default runs only (the profiling
+    # command lines pass --only, --chains or --no-graphs and keep their traces
+    # to the headline kernel).
+    if not args.no_reference_mode and args.only == "all" and not args.no_graphs and \
+            args.chains == 0:
+        result["reference_mode"] = bench_reference_mode(args, G, np)
     if args.copy_mib > 0:
         nbytes = args.copy_mib << 20
         a = ctx.zeros(nbytes // 8)
