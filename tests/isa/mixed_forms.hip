@@ -3,7 +3,8 @@
 // the tile kernel's batches, so that tests/test_mixed_forms_isa.py can count
 // the VALU instructions per list entry each form needs on gfx950.
 //   k_f64      the FP64 tile kernel's form (unit force scalar, 24-B entries)
-//   k_current  the mixed kernel's form (FP64 differences and r2, FP32 scalar)
+//   k_current  the mixed kernel's form (FP64 differences and r2, the FP32
+//              scalar (24 r^6 - 48) / r^14 with the hardware reciprocal)
 //   k_int32_exact_cut  int32 fixed-point tile-relative coordinates (16-B
 //              entries; exact integer differences), exact int64 r2 for the
 //              cutoff, FP32 r2 and scalar, FP64 accumulation
@@ -18,8 +19,7 @@ extern __shared__ char lds[];
 __device__ __forceinline__ float df32(float r2) {
   const float r6 = r2 * r2 * r2;
   const float r14 = r6 * r6 * r2;
-  float inv = __builtin_amdgcn_rcpf(r14);
-  inv = fmaf(fmaf(-r14, inv, 1.0f), inv, inv);
+  const float inv = __builtin_amdgcn_rcpf(r14);
   return fmaf(24.0f, r6, -48.0f) * inv;
 }
 

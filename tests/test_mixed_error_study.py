@@ -11,7 +11,10 @@ operation correctly rounded, a fused multiply-add rounded once):
 
 - "current": the shipped mixed kernel. Differences and r2 in FP64 from the
   FP64 positions, r2 rounded once to FP32, df in FP32, F += df * d in FP64
-  (or d rounded to FP32 and Kahan-compensated FP32 sums).
+  (or d rounded to FP32 and Kahan-compensated FP32 sums). The kernel's
+  FP32 reciprocal is the hardware one (within 1 ulp); the "reciprocal +-1
+  ulp" rows bound it by a reciprocal off by one ulp in the same direction
+  for every pair.
 - "fp32_tile": positions relative to their tile's reference point rounded
   to FP32 (12-byte LDS entries), differences, r2 and F in FP32 arithmetic,
   FP64 accumulation.
@@ -125,6 +128,17 @@ class _Setup:
         out["current, FP64 accumulation"] = self.err(self.acc64(df, self.d64))
         out["current, Kahan FP32 accumulation"] = self.err(
             self.acc_kahan32(df, self.d64.astype(F32)))
+        # The kernel's reciprocal is the hardware one (within 1 ulp, not
+        # refined): bound it by a reciprocal off by one ulp in the same
+        # direction for every pair.
+        r2f = self.r2_64.astype(F32)
+        r6 = r2f * r2f * r2f
+        r14 = r6 * r6 * r2f
+        for tag, toward in (("+1", F32(np.inf)), ("-1", F32(0))):
+            inv = np.nextafter(F32(1.0) / r14, toward)
+            dfb = _fma32(np.full_like(r6, 24.0), r6, np.full_like(r6, -48.0)) * inv
+            dfb = np.where(self.inside, dfb, F32(0))
+            out[f"current, reciprocal {tag} ulp"] = self.err(self.acc64(dfb, self.d64))
         # FP32 tile-relative positions
         d32 = self.dj.astype(F32) - self.di.astype(F32)
         r2 = _r2_32(d32)
@@ -169,6 +183,8 @@ def test_current_form_meets_the_bound(study):
     rho, res = study
     assert res["current, FP64 accumulation"][0] < BOUND[rho]
     assert res["current, Kahan FP32 accumulation"][0] < BOUND[rho]
+    assert res["current, reciprocal +1 ulp"][0] < BOUND[rho]
+    assert res["current, reciprocal -1 ulp"][0] < BOUND[rho]
 
 
 def test_exact_integer_r2_meets_the_bound(study):
