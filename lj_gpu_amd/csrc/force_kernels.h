@@ -299,14 +299,6 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
 // One batch of a tile row: all U partner positions are read from LDS before
 // any is used, then the U entries are computed; the offsets buffer `o` is
 // refilled in between with the batch two ahead (loads from global memory
-// that then have two batches of arithmetic to land). The offset array is
-// padded so these loads stay inside it past the end of the last row; what
-// they read beyond a row is never used. LIMIT masks entries at or beyond
-// `np` (the last, partial batch of a row): their offsets are replaced by 0
-// (a valid slot) and their force by 0.
-// One batch of a tile row: all U partner positions are read from LDS before
-// any is used, then the U entries are computed; the offsets buffer `o` is
-// refilled in between with the batch two ahead (loads from global memory
 // that then have two batches of arithmetic to land). The offset arrays are
 // padded so these loads stay inside them past the end of the last row; what
 // they read beyond a row is never used. LIMIT masks entries at or beyond
@@ -369,28 +361,38 @@ __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restr
   }
 }
 
+// The first batch's offsets of a row in either layout of tile_row_stream,
+// loaded before the tile's staging so that their latency hides behind it.
+template <int TPI, int U, bool SL>
+__device__ __forceinline__ void row_prefetch(const uint16_t* __restrict__ lrow, int sub,
+                                             uint32_t (&oa)[U]) {
+  constexpr int US = SL ? 64 : TPI;
+  const int ls = SL ? 0 : sub;
+#pragma unroll
+  for (int u = 0; u < U; u++) oa[u] = lrow[u * US + ls];
+}
+
 // As tile_row, with the row's entry offsets read from global memory through
-// two alternating offset buffers, each refilled two batches ahead, and the
-// row's last partial batch run as one masked batch. SL selects the offset
-// layout: false, the row's own offsets (`lrow` at the row's first entry,
-// entry k at k); true, the wavefront slices of dev_slice_tiles (`lrow` at
-// the lane's place in its wavefront's slice, batch j slot u at
+// two alternating offset buffers (oa: the first batch, loaded by the caller
+// with row_prefetch; ob: the second), each refilled two batches ahead, and
+// the row's last partial batch run as one masked batch. SL selects the
+// offset layout: false, the row's own offsets (`lrow` at the row's first
+// entry, entry k at k); true, the wavefront slices of dev_slice_tiles
+// (`lrow` at the lane's place in its wavefront's slice, batch j slot u at
 // j * 64 * U + u * 64), so each offset load reads 64 consecutive values.
 template <int TPI, int U, int PREC, bool MI, bool SL, class M>
 __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __restrict__ lrow,
                                                 int np, int sub, double xi, double yi, double zi,
-                                                const PairParams& pp, Accum<PREC>& acc) {
+                                                const PairParams& pp, Accum<PREC>& acc,
+                                                uint32_t (&oa)[U]) {
   constexpr int kB = U * TPI;
   constexpr int JS = SL ? 64 * U : kB;  // layout stride of a batch
   constexpr int US = SL ? 64 : TPI;     // layout stride of a slot
   const int ls = SL ? 0 : sub;
   const int nfull = np / kB;
-  uint32_t oa[U], ob[U];
+  uint32_t ob[U];
 #pragma unroll
-  for (int u = 0; u < U; u++) {
-    oa[u] = lrow[u * US + ls];
-    ob[u] = lrow[JS + u * US + ls];
-  }
+  for (int u = 0; u < U; u++) ob[u] = lrow[JS + u * US + ls];
   int b = 0;
   for (; b + 2 <= nfull; b += 2) {
     tile_batch<TPI, U, PREC, MI, false, true, US>(mom, lrow, oa, b * kB, (b + 2) * JS, ls, np,
@@ -466,6 +468,30 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   double cx = 0.0, cy = 0.0, cz = 0.0;
   // The block's middle particle (its slot when the arrays are in slot order).
   if constexpr (PBC) pos.ld(member ? d1.w : i0 + ni / 2, cx, cy, cz);
+  // The thread's own row, requested before the staging so that its latency
+  // overlaps it: first the indices (own particle, row bounds, offset slice),
+  // then what they address (own position, the row's first offset batch).
+  const int seg = static_cast<int>(threadIdx.x) / TPI;
+  const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
+  const bool active = seg < ni;
+  const int slot = i0 + (active ? seg : 0);
+  const int i = member ? member[slot] : slot;
+  const int sp = slot_ptr[slot];
+  const int sp1 = slot_ptr[slot + 1];  // slot + 1 <= n: slot_ptr has n + 1 entries
+  const int wo = SL && !OFFS_LDS ? wave_off[static_cast<int64_t>(b) * wpb + threadIdx.x / 64] : 0;
+  double xi, yi, zi;
+  pos.ld(i, xi, yi, zi);
+  const int np = active ? sp1 - sp : 0;
+  const int rs = active ? sp - eoff : 0;
+  const uint16_t* grow = nullptr;
+  uint32_t oa[U];
+  if constexpr (!OFFS_LDS) {
+    // Inactive lanes read the first batch too: a slice's first batch, and
+    // the U * TPI entries from the block's first, lie inside the padded
+    // offset arrays.
+    grow = SL ? sliced + wo + (threadIdx.x & 63) : local + eoff + rs;
+    row_prefetch<TPI, U, SL>(grow, sub, oa);
+  }
   // Staging issues all of a thread's loads of a pass before its LDS writes,
   // so each thread has kStageR independent gathers in flight.
   constexpr int kStageR = 4;
@@ -513,21 +539,11 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
     }
   }
   const uint16_t* loff = reinterpret_cast<const uint16_t*>(lb + off_area) + (eoff - e0);
-  const int seg = static_cast<int>(threadIdx.x) / TPI;
-  const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
-  const bool active = seg < ni;
-  const int slot = i0 + (active ? seg : 0);
-  const int i = member ? member[slot] : slot;
-  double xi, yi, zi;
-  pos.ld(i, xi, yi, zi);
   int far = 0;
   if constexpr (PBC) {
     unwrap_near<PBC>(xi, yi, zi, cx, cy, cz, pp);
     far = active && (fabs(xi - cx) >= reach || fabs(yi - cy) >= reach || fabs(zi - cz) >= reach);
   }
-  const int sp = slot_ptr[slot];
-  const int np = active ? slot_ptr[slot + 1] - sp : 0;
-  const int rs = active ? sp - eoff : 0;
   // The frame check ORed over the workgroup through one LDS word: threads
   // that fail it write this tile's index, which no earlier tile of this
   // workgroup used, so the word needs no reset.
@@ -542,14 +558,10 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
       else
         tile_row<TPI, U, PREC, false>(mom, loff + rs, np, sub, xi, yi, zi, pp, acc);
     } else {
-      const uint16_t* grow =
-          SL ? sliced + wave_off[static_cast<int64_t>(b) * wpb + threadIdx.x / 64] +
-                   (threadIdx.x & 63)
-             : local + eoff + rs;
       if (PBC && any_far)
-        tile_row_stream<TPI, U, PREC, true, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, true, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
       else
-        tile_row_stream<TPI, U, PREC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc);
+        tile_row_stream<TPI, U, PREC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
     }
   }
   if constexpr (PREC == kEnergy) {
