@@ -469,16 +469,27 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   // The block's middle particle (its slot when the arrays are in slot order).
   if constexpr (PBC) pos.ld(member ? d1.w : i0 + ni / 2, cx, cy, cz);
   // The thread's own row, requested before the staging so that its latency
-  // overlaps it: first the indices (own particle, row bounds, offset slice),
-  // then what they address (own position, the row's first offset batch).
+  // overlaps it: first the indices (the first staging pass's particles, own
+  // particle, row bounds, offset slice), then what they address (own
+  // position, the row's first offset batch).
+  constexpr int kStageR = 4;
+  const int nt = static_cast<int>(blockDim.x);
   const int seg = static_cast<int>(threadIdx.x) / TPI;
   const int sub = static_cast<int>(threadIdx.x) & (TPI - 1);
   const bool active = seg < ni;
   const int slot = i0 + (active ? seg : 0);
+  int jj[kStageR];
+#pragma unroll
+  for (int r = 0; r < kStageR; r++) {
+    const int t = r * nt + static_cast<int>(threadIdx.x);
+    jj[r] = t < ns ? stage[soff + t] : i0;
+  }
   const int i = member ? member[slot] : slot;
   const int sp = slot_ptr[slot];
   const int sp1 = slot_ptr[slot + 1];  // slot + 1 <= n: slot_ptr has n + 1 entries
-  const int wo = SL && !OFFS_LDS ? wave_off[static_cast<int64_t>(b) * wpb + threadIdx.x / 64] : 0;
+  // The wavefront's index, made scalar: its slice offset is one scalar load.
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / 64);
+  const int wo = SL && !OFFS_LDS ? wave_off[static_cast<int64_t>(b) * wpb + wave] : 0;
   double xi, yi, zi;
   pos.ld(i, xi, yi, zi);
   const int np = active ? sp1 - sp : 0;
@@ -493,15 +504,15 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
     row_prefetch<TPI, U, SL>(grow, sub, oa);
   }
   // Staging issues all of a thread's loads of a pass before its LDS writes,
-  // so each thread has kStageR independent gathers in flight.
-  constexpr int kStageR = 4;
-  const int nt = static_cast<int>(blockDim.x);
+  // so each thread has kStageR independent gathers in flight (the first
+  // pass's indices were loaded above).
   for (int t0 = 0; t0 < ns; t0 += kStageR * nt) {
-    int jj[kStageR];
+    if (t0 > 0) {
 #pragma unroll
-    for (int r = 0; r < kStageR; r++) {
-      const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
-      jj[r] = t < ns ? stage[soff + t] : i0;
+      for (int r = 0; r < kStageR; r++) {
+        const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
+        jj[r] = t < ns ? stage[soff + t] : i0;
+      }
     }
     double x[kStageR], y[kStageR], z[kStageR];
 #pragma unroll
