@@ -504,30 +504,34 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
     row_prefetch<TPI, U, SL>(grow, sub, oa);
   }
   // Staging issues all of a thread's loads of a pass before its LDS writes,
-  // so each thread has kStageR independent gathers in flight (the first
-  // pass's indices were loaded above).
-  for (int t0 = 0; t0 < ns; t0 += kStageR * nt) {
-    if (t0 > 0) {
-#pragma unroll
-      for (int r = 0; r < kStageR; r++) {
-        const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
-        jj[r] = t < ns ? stage[soff + t] : i0;
-      }
-    }
-    double x[kStageR], y[kStageR], z[kStageR];
-#pragma unroll
-    for (int r = 0; r < kStageR; r++) pos.ld(jj[r], x[r], y[r], z[r]);
+  // so each thread has kStageR independent gathers in flight. The first
+  // pass (its indices loaded above) runs ahead of the loop, so its gathers
+  // wait for those indices only.
+#define LJGPU_STAGE_PASS(T0)                                                  \
+  {                                                                           \
+    double x[kStageR], y[kStageR], z[kStageR];                                \
+    _Pragma("unroll") for (int r = 0; r < kStageR; r++)                       \
+        pos.ld(jj[r], x[r], y[r], z[r]);                                      \
+    _Pragma("unroll") for (int r = 0; r < kStageR; r++) {                     \
+      const int t = (T0) + r * nt + static_cast<int>(threadIdx.x);            \
+      if (t < ns) {                                                           \
+        unwrap_near<PBC>(x[r], y[r], z[r], cx, cy, cz, pp);                   \
+        lds[3 * t] = x[r];                                                    \
+        lds[3 * t + 1] = y[r];                                                \
+        lds[3 * t + 2] = z[r];                                                \
+      }                                                                       \
+    }                                                                         \
+  }
+  LJGPU_STAGE_PASS(0)
+  for (int t0 = kStageR * nt; t0 < ns; t0 += kStageR * nt) {
 #pragma unroll
     for (int r = 0; r < kStageR; r++) {
       const int t = t0 + r * nt + static_cast<int>(threadIdx.x);
-      if (t < ns) {
-        unwrap_near<PBC>(x[r], y[r], z[r], cx, cy, cz, pp);
-        lds[3 * t] = x[r];
-        lds[3 * t + 1] = y[r];
-        lds[3 * t + 2] = z[r];
-      }
+      jj[r] = t < ns ? stage[soff + t] : i0;
     }
+    LJGPU_STAGE_PASS(t0)
   }
+#undef LJGPU_STAGE_PASS
   // The block's entry offsets, copied as 16-byte chunks from the 8-entry
   // boundary at or below eoff (the device array is padded by 8 entries).
   const int e0 = eoff & ~7;
@@ -596,12 +600,15 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   }
 }
 
-// Register budget (measured at N = 119,164 on MI355X): the FP64 kernel fits
-// six waves per SIMD (at most 80 VGPRs, no spill) and runs 0.6 % faster
-// there than at five; the mixed kernels keep five (96 VGPRs), where they run
-// 7 % faster than at six and 5 % faster than at four.
+// Register budget (measured on MI355X): five waves per SIMD (at most 96
+// VGPRs) except for FP64 in an open box, which fits six (80 VGPRs) without a
+// spill. With the first staging pass peeled off its loop, the periodic FP64
+// kernel needs more than 80 VGPRs and runs 1.4 % faster at five waves than
+// the unpeeled one at six (N = 119,164); the mixed kernels run 7 % faster at
+// five than at six and 5 % faster than at four.
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS, bool SL>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(PREC == kF64 ? 6 : 5)))
+__global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(PREC == kF64 && !PBC ? 6 : 5)))
 k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ member,
        const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
        const int32_t* __restrict__ stage,
