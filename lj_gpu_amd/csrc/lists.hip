@@ -804,6 +804,8 @@ __device__ __forceinline__ uint32_t tile_hash(int32_t key) {
   return (static_cast<uint32_t>(key) * 2654435761u) >> (32 - kTileHashBits);
 }
 
+// Wavefronts per tile of the sliced offsets: at most 1024 threads.
+constexpr int kSliceWavesMax = 16;
 // A block's rows, staged in LDS: slot i0 + s holds particle mem[s], whose
 // row is list[src[s] ..) and covers entries [beg[s], beg[s + 1]) of the
 // block (beg[ni] = ne).
@@ -1185,29 +1187,34 @@ __global__ void k_wave_slices(const int32_t* __restrict__ desc, int32_t nblocks,
 // k = j * lanes * unroll + u * lanes + l % lanes of the block's row
 // w * (64 / lanes) + l / lanes, or 0 past the row's end (padding). The
 // writes are contiguous; the reads gather short runs of `local` (the
-// block's rows, a few KB, served by the cache). Row starts come from LDS.
+// block's rows, a few KB, served by the cache). Row starts and the block's
+// slice offsets come from LDS; lanes and unroll are powers of two, so the
+// position arithmetic is shifts and masks.
 __global__ __launch_bounds__(kBlock) void k_slice_gather(
     const int32_t* __restrict__ desc, int32_t nblocks, int32_t wpb,
     const int32_t* __restrict__ slot_ptr, const uint16_t* __restrict__ local, int32_t lanes,
     int32_t unroll, const int32_t* __restrict__ wave_off, uint16_t* __restrict__ sliced) {
   __shared__ int32_t rs[kTileRowsMax + 1];
-  const int32_t rpw = 64 / lanes, span = 64 * unroll;
+  __shared__ int32_t wo[kSliceWavesMax + 1];
+  const int32_t lsh = 31 - __clz(lanes), ush = 31 - __clz(unroll);
+  const int32_t bsh = 6 + ush;  // log2 of a batch's 64 * unroll positions
   for (int32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     const int32_t i0 = desc[8 * static_cast<int64_t>(b)], ni = desc[8 * static_cast<int64_t>(b) + 1];
     for (int32_t r = threadIdx.x; r <= ni; r += kBlock) rs[r] = slot_ptr[i0 + r];
+    for (int32_t w = threadIdx.x; w <= wpb; w += kBlock)
+      wo[w] = wave_off[static_cast<int64_t>(b) * wpb + w];
     __syncthreads();
-    const int32_t* wo = wave_off + static_cast<int64_t>(b) * wpb;
-    const int32_t p0 = wo[0], p1 = wo[wpb];  // the block's slices, back to back
-    for (int32_t p = p0 + static_cast<int32_t>(threadIdx.x); p < p1; p += kBlock) {
-      int32_t w = 0;
-      while (w + 1 < wpb && wo[w + 1] <= p) w++;
-      const int32_t off = p - wo[w];
-      const int32_t j = off / span, u = (off % span) >> 6, l = off & 63;
-      const int32_t row = w * rpw + l / lanes;
-      const int32_t k = j * lanes * unroll + u * lanes + l % lanes;
-      uint16_t v = 0;
-      if (row < ni && k < rs[row + 1] - rs[row]) v = local[rs[row] + k];
-      sliced[p] = v;
+    for (int32_t w = 0; w < wpb; w++) {
+      const int32_t p0 = wo[w], len = wo[w + 1] - p0;
+      const int32_t row0 = w << (6 - lsh);
+      for (int32_t off = threadIdx.x; off < len; off += kBlock) {
+        const int32_t j = off >> bsh, u = (off >> 6) & (unroll - 1), l = off & 63;
+        const int32_t row = row0 + (l >> lsh);
+        const int32_t k = (j << (lsh + ush)) + (u << lsh) + (l & (lanes - 1));
+        uint16_t v = 0;
+        if (row < ni && k < rs[row + 1] - rs[row]) v = local[rs[row] + k];
+        sliced[p0 + off] = v;
+      }
     }
     __syncthreads();
   }
@@ -1943,8 +1950,11 @@ int dev_make_tiles(hipStream_t s, const DevCSR& c, int32_t T, const std::vector<
 int dev_slice_tiles(hipStream_t s, const int32_t* slot_ptr, int32_t lanes, int32_t unroll,
                     ListWork& w, DevTiles& t) {
   t.sliced_lanes = t.sliced_unroll = t.waves_per_block = 0;
-  if (t.blocks <= 0 || lanes < 1 || lanes > 64 || (64 % lanes) != 0 || unroll < 1) return LJGPU_OK;
+  if (t.blocks <= 0 || lanes < 1 || lanes > 64 || (64 % lanes) != 0 || unroll < 1 ||
+      (unroll & (unroll - 1)) != 0)
+    return LJGPU_OK;
   const int32_t wpb = (t.tile_particles * lanes + 63) / 64;
+  if (wpb > kSliceWavesMax) return LJGPU_OK;
   const int64_t nw = static_cast<int64_t>(t.blocks) * wpb;
   LCHECK(dev_reserve(w.counts, sizeof(int32_t) * (static_cast<size_t>(nw) + 1), s));
   LCHECK(dev_reserve(t.wave_off, sizeof(int32_t) * (static_cast<size_t>(nw) + 1), s));
