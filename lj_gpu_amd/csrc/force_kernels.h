@@ -311,7 +311,8 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
 // reference's `if (r2 > CL2) continue;` for a whole batch. Rows hold their
 // entries in cutoff classes (cut_partition), so the entries beyond the
 // cutoff collect at the rows' ends. In FP64 the force scalar omits the
-// factor 24*dt (lj_df_unit_f64), which tile_body applies to the row's sum.
+// factor 24*dt (lj_df_unit_f64), which tile_body applies to the row's sum,
+// and the batch's U reciprocals come from one (rcp_f64_batch).
 template <int TPI, int U, int PREC, bool MI, bool LIMIT, bool REFILL, int US, class M>
 __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restrict__ lrow,
                                            uint32_t (&o)[U], int base, int refill, int ls,
@@ -346,11 +347,16 @@ __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restr
   if constexpr (LIMIT) {
     if (!__any(inside)) return;
   }
+  double s[U];
+  if constexpr (PREC == kF64) {
+#pragma unroll
+    for (int u = 0; u < U; u++) s[u] = LIMIT && !valid[u] ? 1.0 : r2[u];
+    rcp_f64_batch<U>(s);
+  }
 #pragma unroll
   for (int u = 0; u < U; u++) {
     if constexpr (PREC == kF64) {
-      acc.add(lj_df_unit_f64(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]), 0.0f, dx[u],
-              dy[u], dz[u]);
+      acc.add(lj_df_unit_f64(s[u], in[u]), 0.0f, dx[u], dy[u], dz[u]);
     } else if constexpr (PREC == kEnergy) {
       acc.add_pair(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]);
     } else {
@@ -600,15 +606,15 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
   }
 }
 
-// Register budget (measured on MI355X): five waves per SIMD (at most 96
-// VGPRs) except for FP64 in an open box, which fits six (80 VGPRs) without a
-// spill. With the first staging pass peeled off its loop, the periodic FP64
-// kernel needs more than 80 VGPRs and runs 1.4 % faster at five waves than
-// the unpeeled one at six (N = 119,164); the mixed kernels run 7 % faster at
-// five than at six and 5 % faster than at four.
+// Register budget (measured on MI355X): five waves per SIMD, at most 96
+// VGPRs (the default FP64 kernel takes 90, the mixed one 90, neither
+// spills). With the first staging pass peeled off its loop and the batched
+// reciprocal, FP64 no longer fits six waves' 80 VGPRs without spilling; the
+// mixed kernels run 7 % faster at five waves than at six and 5 % faster
+// than at four.
 template <int TPI, int U, int PREC, class P, class M, bool PBC, bool OFFS_LDS, bool SL>
 __global__ void __launch_bounds__(1024)
-__attribute__((amdgpu_waves_per_eu(PREC == kF64 && !PBC ? 6 : 5)))
+__attribute__((amdgpu_waves_per_eu(5)))
 k_tile(P pos, M mom, const int32_t* __restrict__ nop, const int32_t* __restrict__ member,
        const int32_t* __restrict__ slot_ptr, const int4* __restrict__ desc,
        const int32_t* __restrict__ stage,
