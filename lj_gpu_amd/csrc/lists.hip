@@ -499,30 +499,44 @@ __global__ void k_cell_order(int64_t ncell, const int32_t* __restrict__ cell_sta
 }
 
 // Row r of a slab (src + r * stride, len[r] entries) to dst + start[r], in
-// order; one wavefront per row.
+// order. One wavefront copies kSlabRows rows at a time, 128 entries of each
+// per pass, every load of a pass issued before its stores (rows are about
+// as long as the search sphere holds partners, so one or two passes).
+constexpr int kSlabRows = 4;
 __global__ __launch_bounds__(kBlock) void k_copy_slab(int32_t n, const int32_t* __restrict__ len,
                                                       const int32_t* __restrict__ start,
                                                       const int32_t* __restrict__ src,
                                                       int64_t stride, int32_t* __restrict__ dst) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
-  for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6); r < n;
-       r += waves) {
-    const int32_t m = len[r];
-    const int32_t* in = src + r * stride;
-    int32_t* o = dst + start[r];
-    // Four loads in flight before their stores.
-    for (int32_t k0 = 0; k0 < m; k0 += 4 * 64) {
-      int32_t v[4];
+  for (int64_t r0 = (blockIdx.x * static_cast<int64_t>(kBlock / 64) + (threadIdx.x >> 6)) *
+                    kSlabRows;
+       r0 < n; r0 += waves * kSlabRows) {
+    int32_t m[kSlabRows], mx = 0;
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int32_t k = k0 + u * 64 + lane;
-        v[u] = k < m ? in[k] : 0;
+    for (int k = 0; k < kSlabRows; k++) {
+      m[k] = r0 + k < n ? len[r0 + k] : 0;
+      mx = max(mx, m[k]);
+    }
+    for (int32_t e0 = 0; e0 < mx; e0 += 2 * 64) {
+      int32_t v[kSlabRows][2];
+#pragma unroll
+      for (int k = 0; k < kSlabRows; k++) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int32_t e = e0 + u * 64 + lane;
+          v[k][u] = e < m[k] ? src[(r0 + k) * stride + e] : 0;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int32_t k = k0 + u * 64 + lane;
-        if (k < m) o[k] = v[u];
+      for (int k = 0; k < kSlabRows; k++) {
+        if (e0 >= m[k]) continue;
+        int32_t* o = dst + start[r0 + k];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int32_t e = e0 + u * 64 + lane;
+          if (e < m[k]) o[e] = v[k][u];
+        }
       }
     }
   }
@@ -1652,7 +1666,7 @@ int dev_makepair(hipStream_t s, const PosIn& q, int32_t n, double box, double se
   if (cap > 0 && mx <= cap) {
     w.last_max_row = mx;
     if (!sort_rows) {
-      k_copy_slab<<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(
+      k_copy_slab<<<grid_for((n + kSlabRows - 1) / kSlabRows, kBlock / 64), kBlock, 0, s>>>(
           n, out.nop.as<int32_t>(), out.ptr.as<int32_t>(), w.tmp.as<int32_t>(), cap,
           out.list.as<int32_t>());
       LTRY(hipGetLastError(), "slab copy kernel");
