@@ -118,6 +118,27 @@ def test_f64_tile_persistent_grid(ctx, k, rho, resident, stage_offsets):
     assert normwise(p, oracle_momenta(k, rho)) < FP64_TOL
 
 
+@pytest.mark.parametrize("k,rho", [(12, 1.0), (31, 0.5)])
+@pytest.mark.parametrize("lanes,tile,unroll,mixed", [(4, 16, 4, None), (4, 16, 2, None),
+                                                     (2, 32, 4, None),
+                                                     (4, 16, 4, G.ACCUM_F64)])
+def test_tile_staging_in_several_passes(ctx, k, rho, lanes, tile, unroll, mixed):
+    """Tiles that stage more positions than one pass covers (4 per thread):
+    the first pass runs ahead of the staging loop, the others in it."""
+    q, box, _, full = periodic(k, rho)
+    if mixed is None:
+        p = run_aos(ctx, q, full, box, lanes=lanes, unroll=unroll, tile=tile)
+        ref, tol = oracle_momenta(k, rho), FP64_TOL
+    else:
+        p = run_aos(ctx, q, full, box, dt=1.0, steps=1, lanes=lanes, unroll=unroll, tile=tile,
+                    mixed=mixed)
+        ref, tol = oracle_force(k, rho), MIXED_TOL[rho]
+    info = ctx.list_info()
+    threads = (info.tile_particles * lanes + 63) // 64 * 64
+    assert info.tile_max_stage > 4 * threads
+    assert normwise(p, ref) < tol
+
+
 @pytest.mark.parametrize("tile", [16, 32, 128, 1024])
 def test_f64_tile_sizes(ctx, tile):
     q, box, _, full = periodic(12, 1.0)
