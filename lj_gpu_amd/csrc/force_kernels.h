@@ -312,8 +312,10 @@ __device__ __forceinline__ void tile_row(const M& mom, const uint16_t* lrow, int
 // entries in cutoff classes (cut_partition), so the entries beyond the
 // cutoff collect at the rows' ends. In FP64 the force scalar omits the
 // factor 24*dt (lj_df_unit_f64), which tile_body applies to the row's sum,
-// and the batch's U reciprocals come from one (rcp_f64_batch).
-template <int TPI, int U, int PREC, bool MI, bool LIMIT, bool REFILL, int US, class M>
+// and the batch's U reciprocals come from one (rcp_f64_batch). PBC is the
+// kernel's box (periodic or open); MI whether this row needs the minimum
+// image.
+template <int TPI, int U, int PREC, bool PBC, bool MI, bool LIMIT, bool REFILL, int US, class M>
 __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restrict__ lrow,
                                            uint32_t (&o)[U], int base, int refill, int ls,
                                            int np, int sub, double xi, double yi, double zi,
@@ -361,6 +363,11 @@ __device__ __forceinline__ void tile_batch(const M& mom, const uint16_t* __restr
       acc.add_pair(rcp_f64(LIMIT && !valid[u] ? 1.0 : r2[u]), in[u]);
     } else {
       float dff = lj_df_mixed(r2[u], pp);
+      // Keeps the cutoff select on the FP32 value (one instruction): the
+      // compiler would otherwise move it past the conversion to FP64, where
+      // a select takes two. Emits nothing. Periodic boxes only: it pushes the
+      // open box's kernels into spilling.
+      if constexpr (PBC) asm("" : "+v"(dff));
       if constexpr (LIMIT) dff = valid[u] ? dff : 0.0f;
       acc.add(static_cast<double>(dff), dff, dx[u], dy[u], dz[u]);
     }
@@ -386,7 +393,7 @@ __device__ __forceinline__ void row_prefetch(const uint16_t* __restrict__ lrow, 
 // entry, entry k at k); true, the wavefront slices of dev_slice_tiles
 // (`lrow` at the lane's place in its wavefront's slice, batch j slot u at
 // j * 64 * U + u * 64), so each offset load reads 64 consecutive values.
-template <int TPI, int U, int PREC, bool MI, bool SL, class M>
+template <int TPI, int U, int PREC, bool PBC, bool MI, bool SL, class M>
 __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __restrict__ lrow,
                                                 int np, int sub, double xi, double yi, double zi,
                                                 const PairParams& pp, Accum<PREC>& acc,
@@ -401,21 +408,21 @@ __device__ __forceinline__ void tile_row_stream(const M& mom, const uint16_t* __
   for (int u = 0; u < U; u++) ob[u] = lrow[JS + u * US + ls];
   int b = 0;
   for (; b + 2 <= nfull; b += 2) {
-    tile_batch<TPI, U, PREC, MI, false, true, US>(mom, lrow, oa, b * kB, (b + 2) * JS, ls, np,
-                                                  sub, xi, yi, zi, pp, acc);
-    tile_batch<TPI, U, PREC, MI, false, true, US>(mom, lrow, ob, (b + 1) * kB, (b + 3) * JS, ls,
-                                                  np, sub, xi, yi, zi, pp, acc);
+    tile_batch<TPI, U, PREC, PBC, MI, false, true, US>(mom, lrow, oa, b * kB, (b + 2) * JS, ls,
+                                                       np, sub, xi, yi, zi, pp, acc);
+    tile_batch<TPI, U, PREC, PBC, MI, false, true, US>(mom, lrow, ob, (b + 1) * kB, (b + 3) * JS,
+                                                       ls, np, sub, xi, yi, zi, pp, acc);
   }
   if (b < nfull) {
-    tile_batch<TPI, U, PREC, MI, false, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi, yi,
-                                                   zi, pp, acc);
+    tile_batch<TPI, U, PREC, PBC, MI, false, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi,
+                                                        yi, zi, pp, acc);
     b++;
 #pragma unroll
     for (int u = 0; u < U; u++) oa[u] = ob[u];
   }
   if (b * kB < np)
-    tile_batch<TPI, U, PREC, MI, true, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi, yi,
-                                                  zi, pp, acc);
+    tile_batch<TPI, U, PREC, PBC, MI, true, false, US>(mom, lrow, oa, b * kB, 0, ls, np, sub, xi,
+                                                       yi, zi, pp, acc);
 }
 
 // LDS-tiled full-list kernel: one workgroup per block of consecutive slots
@@ -580,9 +587,9 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
         tile_row<TPI, U, PREC, false>(mom, loff + rs, np, sub, xi, yi, zi, pp, acc);
     } else {
       if (PBC && any_far)
-        tile_row_stream<TPI, U, PREC, true, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
+        tile_row_stream<TPI, U, PREC, PBC, true, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
       else
-        tile_row_stream<TPI, U, PREC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
+        tile_row_stream<TPI, U, PREC, PBC, false, SL>(mom, grow, np, sub, xi, yi, zi, pp, acc, oa);
     }
   }
   if constexpr (PREC == kEnergy) {
