@@ -614,7 +614,7 @@ __device__ __forceinline__ void tile_body(P pos, M mom, const int32_t* __restric
 }
 
 // Register budget (measured on MI355X): five waves per SIMD, at most 96
-// VGPRs (the default FP64 kernel takes 90, the mixed one 90, neither
+// VGPRs (the default FP64 kernel takes 90, the mixed one 86, neither
 // spills). With the first staging pass peeled off its loop and the batched
 // reciprocal, FP64 no longer fits six waves' 80 VGPRs without spilling; the
 // mixed kernels run 7 % faster at five waves than at six and 5 % faster

@@ -506,6 +506,47 @@ def test_slot_order_copies_are_bitwise_per_step(ctx, layout, prec, graphs):
         assert normwise(res[0][:, :3], oracle_momenta(12, 1.0, steps=20)) < FP64_TOL
 
 
+@pytest.mark.parametrize("prec", ["f64", G.ACCUM_F64, G.ACCUM_KAHAN_F32])
+@pytest.mark.parametrize("chains,graphs", [(2, True), (1, True), (1, False)])
+def test_many_step_calls_carry_the_callers_momenta(ctx, prec, chains, graphs):
+    """A call of 16 or more steps over spatially ordered tiles copies q and p
+    into slot order and p back. From nonzero momenta whose fourth value per
+    row no step changes: calls of 37, of 21 + 16 and of 1 step each end bit
+    for bit alike, with the fourth values unchanged and (FP64) the oracle's
+    momenta added to the starting ones."""
+    q, box, _, full = periodic(12, 1.0)
+    p0 = np.random.default_rng(5).standard_normal(q.shape)
+    ctx.set_graphs(graphs)
+    ctx.set_chains(chains)
+    ctx.set_kernel_config(4, 256, 4)
+    ctx.set_tiling(64)
+    ctx.set_half_mode(G.HALF_AUTO)
+    ctx.set_list(full)
+    res = []
+    for plan in ((37,), (21, 16), (1,) * 37):
+        dq, dp = ctx.array(q), ctx.array(p0)
+        for steps in plan:
+            if prec == "f64":
+                ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, box, steps)
+            else:
+                ctx.force_aos_mixed(dq, dp, PBC_DT, G.CL2, box, steps, prec)
+        ctx.sync()
+        np.testing.assert_array_equal(dq.download(), q)
+        res.append(dp.download())
+        dq.free()
+        dp.free()
+    ctx.set_chains(2)
+    ctx.set_graphs(True)
+    assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    for r in res:
+        np.testing.assert_array_equal(r[:, 3], p0[:, 3])
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+    if prec == "f64":
+        ref = p0[:, :3] + oracle_momenta(12, 1.0, steps=37)[:, :3]
+        assert normwise(res[0][:, :3], ref) < FP64_TOL
+
+
 def test_graph_cache_reuses_graphs_across_step_counts_and_precisions(ctx):
     """Calls that alternate step counts and precisions replay cached graphs
     (64/16/4-launch chunks, captured once per argument set) and give the

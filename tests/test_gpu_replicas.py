@@ -177,23 +177,27 @@ def test_batched_rejections(ctx, work):
     ctx.set_replicas([], [])
 
 
-def test_batched_slot_copies_are_bitwise_per_step(ctx, work):
+@pytest.mark.parametrize("lanes,tile", [(8, 32), (4, 64)])
+def test_batched_slot_copies_are_bitwise_per_step(ctx, work, lanes, tile):
     """A 20-step call over replicas runs on slot-ordered copies of q and p;
     20 one-step calls read the caller's arrays through the slot order. Same
-    arithmetic with each replica's own box: equal bit for bit."""
+    arithmetic with each replica's own box: equal bit for bit, from nonzero
+    momenta whose fourth value per row comes back unchanged."""
     q, off, boxes = work
-    configure(ctx)
+    configure(ctx, lanes=lanes, tile=tile)
     ctx.set_replicas(off, boxes)
     dq = ctx.array(q)
     ctx.build_list(dq, q.shape[0], 0.0, G.LIST_FULL)
     assert ctx.get_tile_order()[0] == G.TILE_ORDER_SPATIAL
+    p0 = np.random.default_rng(7).standard_normal(q.shape)
     res = []
     for calls, steps in ((1, 20), (20, 1)):
-        dp = ctx.zeros(q.shape)
+        dp = ctx.array(p0)
         for _ in range(calls):
             ctx.force_aos_f64(dq, dp, PBC_DT, G.CL2, 0.0, steps)
         ctx.sync()
         res.append(dp.download())
     ctx.set_replicas([], [])
     np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0][:, 3], p0[:, 3])
     np.testing.assert_array_equal(dq.download(), q)
