@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """bench.py — Lennard-Jones pair-interaction throughput on one MI355X.
 
-Workload (the north-star size of the reference's force path): FCC lattice of
+Workload (the headline size of the reference's force path): FCC lattice of
 4*31^3 = 119,164 particles at density 1.0 in a periodic cubic box
 (L = 49.21), seeded jitter U(-0.05, 0.05), cutoff 3.0, search length 3.3,
 full Verlet list (16,895,170 entries = 8,447,585 unique pairs), dt = 0.01,

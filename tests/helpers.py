@@ -58,6 +58,17 @@ def oracle_momenta(k: int, density: float, steps: int = STEPS, dt: float = PBC_D
 
 
 @functools.lru_cache(maxsize=None)
+def oracle_force_reference(density: float = 0.5):
+    """Oracle F (one step with dt = 1) of the reference's own problem (open
+    box), from its half list."""
+    q, half = reference_mode(density)
+    csr = O.CSR(half.number_of_partners, half.pointer, half.sorted_list, "half")
+    p = np.zeros_like(q)
+    O.force_aos(csr, q, p, 1.0, O.CL2, 0.0, steps=1, method="sorted")
+    return p
+
+
+@functools.lru_cache(maxsize=None)
 def oracle_force(k: int, density: float):
     """Oracle F (one step with dt = 1) from the full list."""
     q, box, _, full = periodic(k, density)
